@@ -1,0 +1,224 @@
+"""Benchmark of the immersed-boundary transfer hot path on one MI355X.
+
+One step = one interpolation (eulr2lagr) plus one spread (lagr2eulr) over the whole batch,
+replayed as one captured hipGraph with inputs resident in HBM.  Default workload: the
+512 x 256 x 512 channel (h = 1/64, x,z periodic, y walls) with 2000 seeded non-overlapping
+capsules of radius U(4h,10h) and degree 8..24 by radius (1.16 M Lagrangian points).
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--workload uneven|moderate|small]
+
+Prints one JSON line.  `value` = Lagrangian points per second over interp + spread.
+`roofline` is for the dominant kernel (the tile spread kernel, which writes the whole force
+field); `path_roofline` is the same figure for the whole interp + spread step.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+
+
+def support_cells(g, caps):
+    """U: distinct grid cells inside any point's support window [f-1, f+2]^3 (slot-0 copy,
+    x and y wrapped periodically, z clamped to the walls)."""
+    mark = np.zeros(g.itot * g.jtot * g.ktot, dtype=bool)
+    n = caps.npts
+    step = 1 << 16
+    off = np.arange(4)
+    for a in range(0, n, step):
+        b = min(n, a + step)
+        fi = np.floor(caps.x[a:b] * g.dxi).astype(np.int64) - 1
+        fj = np.floor(caps.y[a:b] * g.dyi).astype(np.int64) - 1
+        fk = np.floor(caps.z[a:b] * g.dzi).astype(np.int64) - 1
+        I = (fi[:, None] + off[None, :] - 1) % g.itot
+        J = (fj[:, None] + off[None, :] - 1) % g.jtot
+        K = fk[:, None] + off[None, :]
+        Kok = (K >= 1) & (K <= g.ktot)
+        K = np.clip(K, 1, g.ktot) - 1
+        idx = (I[:, :, None, None] + g.itot * (J[:, None, :, None] + g.jtot * K[:, None, None, :]))
+        ok = np.broadcast_to(Kok[:, None, None, :], idx.shape)
+        mark[idx[ok]] = True
+    return int(mark.sum())
+
+
+def algorithmic_bytes(g, caps, U):
+    n = caps.npts
+    interp = n * 48 + U * 24
+    spread = n * 56 + 24 * g.ncells_halo
+    return interp, spread
+
+
+def cpu_baseline(wl):
+    """The CPU oracle (oracle/ibx_oracle.c: the reference's loops restated in C, gcc -O2, one
+    thread) on the same workload: one eulr2lagr and one lagr2eulr over the whole batch
+    (about 6 s of CPU work for the 2000-capsule batch)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle as O
+
+    g, caps = wl.grid, wl.caps
+    t0 = time.perf_counter()
+    O.eulr2lagr(g, wl.u, wl.v, wl.w, caps.cap_off, caps.x, caps.y, caps.z)
+    t1 = time.perf_counter()
+    O.lagr2eulr(g, caps.cap_off, caps.x, caps.y, caps.z, wl.fx, wl.fy, wl.fz, caps.area)
+    t2 = time.perf_counter()
+    return {
+        "value": caps.npts / (t2 - t0),
+        "unit": "Lag-pts/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"whole batch ({caps.npts} points, {caps.ncap} capsules): one eulr2lagr "
+                   f"({t1 - t0:.2f} s) + one lagr2eulr incl. field zeroing and halos "
+                   f"({t2 - t1:.2f} s), oracle/ibx_oracle.c gcc -O2, 1 thread"),
+    }
+
+
+def read_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), committed
+    under profiles/ as JSON; None if absent."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="uneven", choices=["uneven", "moderate", "small"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-graph", action="store_true", help="eager launches instead of a graph")
+    ap.add_argument("--config", type=int, default=-1,
+                    help="tile configuration id (default: autotune)")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="run warmup + steps only (for rocprofv3), no extras")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU only; the path's multi-GPU sharding is "
+              "described in DESIGN.md and is not run.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from suguru_amd import Batch, IBTransfer, workload as W
+
+    t_setup = time.perf_counter()
+    maker = {"uneven": W.config_uneven, "moderate": W.config_moderate, "small": W.config_small}
+    wl = maker[args.workload]()
+    g, caps = wl.grid, wl.caps
+    dev = torch.device("cuda:0")
+
+    def up(a):
+        return torch.from_numpy(np.ascontiguousarray(np.asarray(a).reshape(-1, order="F"))).to(dev)
+
+    t = {k: up(getattr(wl, k)) for k in ("u", "v", "w", "fx", "fy", "fz")}
+    for k in ("x", "y", "z", "area"):
+        t[k] = up(getattr(caps, k))
+    n = caps.npts
+    for k in ("ul", "vl", "wl"):
+        t[k] = torch.empty(n, dtype=torch.float64, device=dev)
+    for k in ("forcex", "forcey", "forcez"):
+        t[k] = torch.empty(g.ncells_halo, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    ctx = IBTransfer(g, 0)
+    batch = Batch(ctx, caps.cap_off)
+    batch.bind(**t)
+    if args.config >= 0:
+        batch.set_config(args.config)
+        choice = args.config
+    else:
+        cache = os.path.join(os.environ.get("TMPDIR", "/tmp"), "ibx_autotune.txt")
+        choice = batch.autotune(cache)
+    batch.capture(use_graph=not args.no_graph)
+    t_setup = time.perf_counter() - t_setup
+
+    for _ in range(args.warmup):
+        batch.run()
+    ctx.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run()
+    ctx.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms = 1000.0 * elapsed / args.steps
+    if args.profile_only:
+        print(json.dumps({"ms_per_step": ms, "config": Batch.config_names()[choice]}))
+        return
+
+    prof = batch.profile(reps=max(5, args.steps))
+    U = support_cells(g, caps)
+    b_interp, b_spread = algorithmic_bytes(g, caps, U)
+    tile_s = prof["spread_us"] * 1e-6
+    achieved = b_spread / tile_s / 1e9
+    traffic = read_traffic(f"traffic_spread_tile_{args.workload}.json")
+    out = {
+        "metric": "IB interp+spread Lag-pts/s & HBM GB/s; step time surrogate vs plain @1e-3",
+        "value": n / (ms * 1e-3),
+        "unit": "Lag-pts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Poiseuille + divergence-free perturbation; N(0,1) forces; "
+                "seeded non-overlapping spherical capsules)",
+        "config": {
+            "workload": wl.name,
+            "grid_itot_jtot_ktot": [g.itot, g.jtot, g.ktot],
+            "capsules": caps.ncap,
+            "lagrangian_points": n,
+            "volume_fraction": round(W.volume_fraction(g, caps), 4),
+            "tile_config": Batch.config_names()[choice],
+            "graph": not args.no_graph,
+        },
+        "roofline": {
+            "kernel": "k_spread (tile spread, writes the full 3-component force field)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None if traffic is None else traffic.get("bytes_per_launch"),
+            "algorithmic_bytes": b_spread,
+            "avg_launch_us": prof["spread_us"],
+        },
+        "path_roofline": {
+            "achieved": (b_interp + b_spread) / (ms * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (b_interp + b_spread) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "algorithmic_bytes_interp": b_interp,
+            "algorithmic_bytes_spread": b_spread,
+            "support_cells_U": U,
+            "stage_us": prof,
+        },
+        "setup_s": round(t_setup, 2),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(wl)
+        out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    batch.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

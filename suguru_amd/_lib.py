@@ -1,0 +1,115 @@
+"""ctypes binding of libibx.so (the C ABI in include/ibx.h).
+
+The library is the product: if it is missing, loading fails loudly -- there is no CPU
+fallback anywhere in this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libibx.so")
+HEADER = os.path.join(ROOT, "include", "ibx.h")
+
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_int64_p = ctypes.POINTER(ctypes.c_int64)
+
+IBX_OK = 0
+IBX_E_ARG = -1
+IBX_E_HIP = -2
+IBX_E_DOMAIN = -3
+IBX_E_CAPACITY = -4
+IBX_E_STATE = -5
+
+
+class IbxGrid(ctypes.Structure):
+    _fields_ = [
+        ("itot", ctypes.c_int32),
+        ("jtot", ctypes.c_int32),
+        ("ktot", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("lx", ctypes.c_double),
+        ("ly", ctypes.c_double),
+        ("lz", ctypes.c_double),
+        ("dxi", ctypes.c_double),
+        ("dyi", ctypes.c_double),
+        ("dzi", ctypes.c_double),
+        ("dx", ctypes.c_double),
+        ("dy", ctypes.c_double),
+        ("dz", ctypes.c_double),
+        ("boundleft", ctypes.c_double),
+        ("boundfront", ctypes.c_double),
+    ]
+
+
+class IbxBatchBufs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "u", "v", "w", "x", "y", "z", "fx", "fy", "fz", "area",
+        "ul", "vl", "wl", "forcex", "forcey", "forcez")]
+
+
+def grid_struct(g) -> IbxGrid:
+    return IbxGrid(g.itot, g.jtot, g.ktot, 0, g.lx, g.ly, g.lz, g.dxi, g.dyi, g.dzi, g.dx,
+                   g.dy, g.dz, g.boundleft, g.boundfront)
+
+
+_lib = None
+
+
+def declared_symbols(header: str = HEADER):
+    """Names of the functions include/ibx.h declares."""
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ibx_[a-z0-9_]+)\s*\(", text)))
+
+
+def load() -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m suguru_amd.build` "
+            "(there is no fallback implementation)")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    I64 = ctypes.c_int64
+    D = ctypes.c_double
+    sig = {
+        "ibx_create": (ctypes.c_int, [ctypes.POINTER(P), ctypes.POINTER(IbxGrid), ctypes.c_int]),
+        "ibx_destroy": (None, [P]),
+        "ibx_last_error": (ctypes.c_char_p, [P]),
+        "ibx_set_stream": (ctypes.c_int, [P, P]),
+        "ibx_sync": (ctypes.c_int, [P]),
+        "ibx_version": (ctypes.c_char_p, []),
+        "ibx_kernel": (D, [D]),
+        "ibx_eulr2lagr": (ctypes.c_int, [P, P, P, P, I64, P, P, P, P, P, P, P]),
+        "ibx_lagr2eulr": (ctypes.c_int, [P, I64, P, P, P, P, P, P, P, P, P, P, P]),
+        "ibx_lagr2eulrind": (ctypes.c_int, [P, I64, P, P, P, P, P, P, P, P, P, P, P]),
+        "ibx_interp_f64_dev": (ctypes.c_int, [P, P, P, P, I64, P, I64, P, P, P, P, P, P]),
+        "ibx_spread_f64_dev": (ctypes.c_int, [P, I64, P, I64, P, P, P, P, P, P, P, P, P, P]),
+        "ibx_spread_cc_f64_dev": (ctypes.c_int, [P, I64, P, I64, P, P, P, P, P, P, P, P, P, P]),
+        "ibx_update_pos": (ctypes.c_int, [P, I64, P, D, P, P, P, P, P, P, P, P, P, P, P]),
+        "ibx_update_pos_dev": (ctypes.c_int, [P, I64, P, I64, D, P, P, P, P, P, P, P, P, P, P]),
+        "ibx_batch_create": (ctypes.c_int, [P, ctypes.POINTER(P), I64, P]),
+        "ibx_batch_bind": (ctypes.c_int, [P, ctypes.POINTER(IbxBatchBufs)]),
+        "ibx_batch_autotune": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
+        "ibx_batch_set_config": (ctypes.c_int, [P, ctypes.c_int]),
+        "ibx_batch_num_configs": (ctypes.c_int, []),
+        "ibx_batch_config_name": (ctypes.c_char_p, [ctypes.c_int]),
+        "ibx_batch_capture": (ctypes.c_int, [P, ctypes.c_int]),
+        "ibx_batch_run": (ctypes.c_int, [P]),
+        "ibx_batch_profile": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(D),
+                                             ctypes.POINTER(D), ctypes.POINTER(D),
+                                             ctypes.POINTER(D)]),
+        "ibx_batch_destroy": (None, [P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib

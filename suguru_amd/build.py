@@ -1,0 +1,55 @@
+"""Build libibx.so (gfx950) in-tree with hipcc.
+
+    python -m suguru_amd.build            # or __graft_entry__.build()
+
+Flags: -ffp-contract=off keeps every FP64 product and sum separately rounded (the library's
+results are bit-identical to the reference loop order); packed FP32 instructions are
+disabled for the whole object (-Xclang -target-feature -Xclang -packed-fp32-ops).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libibx.so")
+SOURCES = [os.path.join(HERE, "csrc", "ibx.hip")]
+DEPS = SOURCES + [os.path.join(HERE, "csrc", "ibx_device.hpp"), os.path.join(ROOT, "include", "ibx.h")]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("IBX_OFFLOAD_ARCH", "gfx950")
+
+
+def hipcc_cmd(out: str = LIB):
+    return [
+        HIPCC,
+        f"--offload-arch={ARCH}",
+        "-O3",
+        "-std=c++17",
+        "-fPIC",
+        "-shared",
+        "-ffp-contract=off",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-Wall",
+        "-Wno-unused-function",
+        "-Wno-unused-variable",
+        "-o", out,
+        *SOURCES,
+    ]
+
+
+def build(verbose: bool = True) -> str:
+    tmp = LIB + ".tmp"
+    cmd = hipcc_cmd(tmp)
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

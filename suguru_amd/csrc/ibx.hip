@@ -1,0 +1,780 @@
+// ibx.hip -- host runtime and C ABI of libibx (declarations and reference citations in
+// include/ibx.h).  One context per grid and device; all device work is stream-ordered on the
+// context stream.  The host-array entry points stage through context-owned HBM buffers;
+// the *_dev entry points take HBM pointers and return without waiting.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/ibx.h"
+#include "ibx_device.hpp"
+
+#pragma clang fp contract(off)
+
+using namespace ibx;
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+// ---- tile configurations (compile-time instances; the batch autotuner picks one) --------
+using SpreadFn = void (*)(GridD, TileGeom, const unsigned*, unsigned long long*,
+                          unsigned long long, const double*, const double*, const double*,
+                          const double*, const double*, const double*, const double*, double*,
+                          double*, double*);
+
+struct TileCfg {
+  int ti, tj, tk;
+  bool persist;
+  const char* name;
+  SpreadFn plain;
+  SpreadFn cc;
+};
+
+#define IBX_CFG(TI, TJ, TK, P, NAME) \
+  {TI, TJ, TK, P, NAME, &k_spread<TI, TJ, TK, false, P>, &k_spread<TI, TJ, TK, true, P>}
+
+const TileCfg kCfgs[] = {
+    IBX_CFG(16, 8, 8, false, "tile16x8x8"),
+    IBX_CFG(8, 8, 8, false, "tile8x8x8"),
+    IBX_CFG(32, 8, 4, false, "tile32x8x4"),
+    IBX_CFG(16, 16, 4, false, "tile16x16x4"),
+    IBX_CFG(32, 4, 8, false, "tile32x4x8"),
+    IBX_CFG(16, 8, 8, true, "tile16x8x8-persistent"),
+    IBX_CFG(8, 8, 8, true, "tile8x8x8-persistent"),
+    IBX_CFG(32, 8, 4, true, "tile32x8x4-persistent"),
+};
+constexpr int kNumCfgs = (int)(sizeof(kCfgs) / sizeof(kCfgs[0]));
+
+}  // namespace
+
+struct ibx_ctx {
+  ibx_grid grid{};
+  GridD gd{};
+  int device = 0;
+  int num_cus = 256;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int* d_err = nullptr;
+  // spread binning
+  DevBuf tcount, toff, tpart, entries;
+  // staging for the host-array entry points
+  DevBuf s_f1, s_f2, s_f3, s_x, s_y, s_z, s_a1, s_a2, s_a3, s_a4, s_o1, s_o2, s_o3, s_off;
+  int cfg = 0;
+  unsigned long long ecap = 0;  // entries capacity (elements)
+};
+
+struct ibx_batch {
+  ibx_ctx* ctx = nullptr;
+  long long ncap = 0, npts = 0;
+  DevBuf d_off;
+  ibx_batch_bufs bufs{};
+  bool bound = false;
+  int cfg = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool use_graph = false;
+  bool captured = false;
+};
+
+namespace {
+
+int fail(ibx_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                 \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return fail((ctx), IBX_E_HIP,                                                       \
+                  std::string(#call) + ": " + hipGetErrorString(e_) + " (" + __FILE__ + ":" + \
+                      std::to_string(__LINE__) + ")");                                    \
+  } while (0)
+
+int ensure(ibx_ctx* c, DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return IBX_OK;
+  if (b.p) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  size_t nb = std::max<size_t>(bytes, 256);
+  HIPCHK(c, hipMalloc(&b.p, nb));
+  b.bytes = nb;
+  return IBX_OK;
+}
+
+void release(DevBuf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+size_t field_elems(const ibx_grid& g) {
+  return (size_t)(g.itot + 2) * (size_t)(g.jtot + 2) * (size_t)(g.ktot + 2);
+}
+
+TileGeom geom(const ibx_grid& g, const TileCfg& cf, bool cc) {
+  TileGeom t{};
+  t.ti = cf.ti, t.tj = cf.tj, t.tk = cf.tk;
+  t.kbase = cc ? 0 : 1;
+  t.nkp = cc ? g.ktot + 2 : g.ktot;
+  t.klo = cc ? 0 : 1;
+  t.khi = cc ? g.ktot + 1 : g.ktot;
+  t.nti = (g.itot + cf.ti - 1) / cf.ti;
+  t.ntj = (g.jtot + cf.tj - 1) / cf.tj;
+  t.ntk = (t.nkp + cf.tk - 1) / cf.tk;
+  t.ntiles = (long long)t.nti * t.ntj * t.ntk;
+  return t;
+}
+
+inline unsigned blocks_for(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+int check_offsets(ibx_ctx* c, long long ncap, const int64_t* off) {
+  if (ncap < 0) return fail(c, IBX_E_ARG, "ncap < 0");
+  if (ncap > 0 && !off) return fail(c, IBX_E_ARG, "cap_off is NULL");
+  if (ncap == 0) return IBX_OK;
+  if (off[0] != 0) return fail(c, IBX_E_ARG, "cap_off[0] != 0");
+  for (long long p = 0; p < ncap; ++p)
+    if (off[p + 1] < off[p]) return fail(c, IBX_E_ARG, "cap_off is decreasing");
+  if (off[ncap] >= (1ll << 28)) return fail(c, IBX_E_ARG, "more than 2^28 - 1 points");
+  return IBX_OK;
+}
+
+// exclusive scan counts[0..n) -> off[0..n]
+int enqueue_scan(ibx_ctx* c, const unsigned* counts, long long n, unsigned* off) {
+  long long nb = (n + SCAN_B - 1) / SCAN_B;
+  int rc = ensure(c, c->tpart, (size_t)std::max<long long>(nb, 1) * sizeof(unsigned));
+  if (rc) return rc;
+  unsigned* part = (unsigned*)c->tpart.p;
+  k_scan_reduce<<<(unsigned)nb, 256, 0, c->stream>>>(counts, n, part);
+  k_scan_parts<<<1, 256, 0, c->stream>>>(part, nb);
+  k_scan_apply<<<(unsigned)nb, 256, 0, c->stream>>>(counts, n, part, off);
+  HIPCHK(c, hipGetLastError());
+  return IBX_OK;
+}
+
+// The whole spread (binning, scan, fill, tile kernel, wall planes) on the context stream.
+// size_entries: read the candidate count back and grow the list (eager path); otherwise
+// the current capacity is used and overflow is reported through the device error word.
+int enqueue_spread(ibx_ctx* c, int cfgid, bool cc, long long ncap, const long long* d_off,
+                   long long npts, const double* x, const double* y, const double* z,
+                   const double* f1, const double* f2, const double* f3, const double* area,
+                   double* o1, double* o2, double* o3, bool size_entries) {
+  const TileCfg& cf = kCfgs[cfgid];
+  TileGeom tg = geom(c->grid, cf, cc);
+  int rc;
+  if ((rc = ensure(c, c->tcount, (size_t)tg.ntiles * sizeof(unsigned)))) return rc;
+  if ((rc = ensure(c, c->toff, (size_t)(tg.ntiles + 1) * sizeof(unsigned)))) return rc;
+  unsigned* cnt = (unsigned*)c->tcount.p;
+  unsigned* off = (unsigned*)c->toff.p;
+  if (c->ecap == 0) {
+    if ((rc = ensure(c, c->entries, 1024 * sizeof(unsigned long long)))) return rc;
+    c->ecap = c->entries.bytes / sizeof(unsigned long long);
+  }
+  k_fill_u32<<<blocks_for(tg.ntiles, 256), 256, 0, c->stream>>>(cnt, tg.ntiles, 0u);
+  if (npts > 0)
+    k_bin<0><<<blocks_for(npts, 256), 256, 0, c->stream>>>(
+        c->gd, tg, cc, ncap, d_off, npts, x, y, z, cnt, off, (unsigned long long*)c->entries.p,
+        c->ecap, c->d_err);
+  if ((rc = enqueue_scan(c, cnt, tg.ntiles, off))) return rc;
+  if (size_entries) {
+    unsigned total = 0;
+    HIPCHK(c, hipMemcpyAsync(&total, off + tg.ntiles, sizeof(unsigned), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if ((unsigned long long)total > c->ecap) {
+      size_t want = (size_t)total + total / 4 + 1024;
+      if ((rc = ensure(c, c->entries, want * sizeof(unsigned long long)))) return rc;
+      c->ecap = c->entries.bytes / sizeof(unsigned long long);
+    }
+  }
+  k_fill_u32<<<blocks_for(tg.ntiles, 256), 256, 0, c->stream>>>(cnt, tg.ntiles, 0u);
+  if (npts > 0)
+    k_bin<1><<<blocks_for(npts, 256), 256, 0, c->stream>>>(
+        c->gd, tg, cc, ncap, d_off, npts, x, y, z, cnt, off, (unsigned long long*)c->entries.p,
+        c->ecap, c->d_err);
+  SpreadFn fn = cc ? cf.cc : cf.plain;
+  unsigned nblk;
+  if (cf.persist) {
+    long long want = (long long)c->num_cus * 4;
+    nblk = (unsigned)std::max<long long>(1, std::min<long long>(want, tg.ntiles));
+  } else {
+    long long per = (tg.ntiles + 7) / 8;
+    nblk = (unsigned)(per * 8);
+  }
+  fn<<<nblk, SP_THREADS, 0, c->stream>>>(c->gd, tg, off, (unsigned long long*)c->entries.p,
+                                          c->ecap, x, y, z, f1, f2, f3, area, o1, o2, o3);
+  if (!cc) {
+    long long n2 = 2 * c->gd.sz;
+    k_zero_kplanes<<<blocks_for(n2, 256), 256, 0, c->stream>>>(c->gd, o1, o2, o3);
+  }
+  HIPCHK(c, hipGetLastError());
+  return IBX_OK;
+}
+
+int enqueue_interp(ibx_ctx* c, const double* u, const double* v, const double* w,
+                   long long npts, const double* x, const double* y, const double* z,
+                   double* ul, double* vl, double* wl) {
+  if (npts > 0)
+    k_interp<<<blocks_for(npts, 256), 256, 0, c->stream>>>(c->gd, u, v, w, npts, x, y, z, ul,
+                                                            vl, wl, c->d_err);
+  HIPCHK(c, hipGetLastError());
+  return IBX_OK;
+}
+
+// read and clear the device error word (synchronises the stream)
+int collect_errors(ibx_ctx* c) {
+  int h = 0;
+  HIPCHK(c, hipMemcpyAsync(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (h) {
+    k_fill_i32<<<1, 64, 0, c->stream>>>(c->d_err, 1, 0);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (h & ERR_NONFINITE) return fail(c, IBX_E_DOMAIN, "non-finite Lagrangian position");
+    if (h & ERR_CAPACITY)
+      return fail(c, IBX_E_CAPACITY, "candidate list capacity exceeded; re-plan the batch");
+  }
+  return IBX_OK;
+}
+
+int upload(ibx_ctx* c, DevBuf& b, const void* src, size_t bytes) {
+  int rc = ensure(c, b, bytes);
+  if (rc) return rc;
+  if (bytes) HIPCHK(c, hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return IBX_OK;
+}
+
+int download(ibx_ctx* c, void* dst, const DevBuf& b, size_t bytes) {
+  if (bytes) HIPCHK(c, hipMemcpyAsync(dst, b.p, bytes, hipMemcpyDeviceToHost, c->stream));
+  return IBX_OK;
+}
+
+}  // namespace
+
+// ======================================================================================
+// C ABI
+// ======================================================================================
+extern "C" {
+
+const char* ibx_version(void) { return "ibx 0.1 (gfx950, FP64, bit-exact reference order)"; }
+
+double ibx_kernel(double r) {
+  double ar = fabs(r);
+  if (ar > 1.5) return 0.0;
+  if (ar > 0.5) {
+    double t = 1.0 - ar;
+    return (1.0 / 6.0) * ((5.0 - 3.0 * ar) - sqrt(-(3.0 * (t * t)) + 1.0));
+  }
+  return (1.0 / 3.0) * (1.0 + sqrt(-(3.0 * (r * r)) + 1.0));
+}
+
+int ibx_create(ibx_ctx** out, const ibx_grid* g, int device) {
+  if (!out || !g) return IBX_E_ARG;
+  *out = nullptr;
+  if (g->itot < 4 || g->jtot < 4 || g->ktot < 4 || g->reserved != 0) return IBX_E_ARG;
+  if (!(g->dxi > 0 && g->dyi > 0 && g->dzi > 0 && g->lx > 0 && g->ly > 0 && g->lz > 0))
+    return IBX_E_ARG;
+  if ((double)(g->itot + 2) * (g->jtot + 2) * (g->ktot + 2) > 4.0e9) return IBX_E_ARG;
+  ibx_ctx* c = new ibx_ctx();
+  c->grid = *g;
+  c->device = device;
+  GridD& d = c->gd;
+  d.itot = g->itot, d.jtot = g->jtot, d.ktot = g->ktot;
+  d.sy = g->itot + 2;
+  d.sz = (long long)(g->itot + 2) * (g->jtot + 2);
+  d.lx = g->lx, d.ly = g->ly;
+  d.dxi = g->dxi, d.dyi = g->dyi, d.dzi = g->dzi;
+  d.bl = g->boundleft, d.bf = g->boundfront;
+  d.dveul = g->dx * g->dy * g->dz;  // interp_spread.f90:534
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&c->d_err, sizeof(int));
+  if (e == hipSuccess) e = hipMemsetAsync(c->d_err, 0, sizeof(int), c->own);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->own);
+  if (e == hipSuccess) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+      c->num_cus = prop.multiProcessorCount;
+  }
+  if (e != hipSuccess) {
+    if (c->own) (void)hipStreamDestroy(c->own);
+    delete c;
+    return IBX_E_HIP;
+  }
+  c->stream = c->own;
+  *out = c;
+  return IBX_OK;
+}
+
+void ibx_destroy(ibx_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  DevBuf* all[] = {&c->tcount, &c->toff, &c->tpart, &c->entries, &c->s_f1, &c->s_f2,
+                   &c->s_f3,   &c->s_x,  &c->s_y,   &c->s_z,     &c->s_a1, &c->s_a2,
+                   &c->s_a3,   &c->s_a4, &c->s_o1,  &c->s_o2,    &c->s_o3, &c->s_off};
+  for (DevBuf* b : all) release(*b);
+  if (c->d_err) (void)hipFree(c->d_err);
+  if (c->own) (void)hipStreamDestroy(c->own);
+  delete c;
+}
+
+const char* ibx_last_error(const ibx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int ibx_set_stream(ibx_ctx* c, void* s) {
+  if (!c) return IBX_E_ARG;
+  c->stream = s ? (hipStream_t)s : c->own;
+  return IBX_OK;
+}
+
+int ibx_sync(ibx_ctx* c) {
+  if (!c) return IBX_E_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  return collect_errors(c);
+}
+
+// ---- device-pointer entry points --------------------------------------------------------
+
+int ibx_interp_f64_dev(ibx_ctx* c, const double* u, const double* v, const double* w,
+                       int64_t ncap, const int64_t* cap_off, int64_t npts, const double* x,
+                       const double* y, const double* z, double* ul, double* vl, double* wl) {
+  if (!c) return IBX_E_ARG;
+  (void)cap_off;
+  if (ncap < 0 || npts < 0) return fail(c, IBX_E_ARG, "negative size");
+  if (npts >= (1ll << 28)) return fail(c, IBX_E_ARG, "more than 2^28 - 1 points");
+  if (npts > 0 && !(u && v && w && x && y && z && ul && vl && wl))
+    return fail(c, IBX_E_ARG, "null pointer");
+  HIPCHK(c, hipSetDevice(c->device));
+  return enqueue_interp(c, u, v, w, npts, x, y, z, ul, vl, wl);
+}
+
+static int spread_dev(ibx_ctx* c, bool cc, int64_t ncap, const int64_t* cap_off, int64_t npts,
+                      const double* x, const double* y, const double* z, const double* f1,
+                      const double* f2, const double* f3, const double* area, double* o1,
+                      double* o2, double* o3) {
+  if (!c) return IBX_E_ARG;
+  if (ncap < 0 || npts < 0) return fail(c, IBX_E_ARG, "negative size");
+  if (npts >= (1ll << 28)) return fail(c, IBX_E_ARG, "more than 2^28 - 1 points");
+  if (!(o1 && o2 && o3)) return fail(c, IBX_E_ARG, "null output field");
+  if (npts > 0 && !(cap_off && x && y && z && f1 && f2 && f3 && area && ncap > 0))
+    return fail(c, IBX_E_ARG, "null pointer or no capsules");
+  HIPCHK(c, hipSetDevice(c->device));
+  return enqueue_spread(c, c->cfg, cc, ncap, (const long long*)cap_off, npts, x, y, z, f1, f2,
+                        f3, area, o1, o2, o3, true);
+}
+
+int ibx_spread_f64_dev(ibx_ctx* c, int64_t ncap, const int64_t* cap_off, int64_t npts,
+                       const double* x, const double* y, const double* z, const double* fx,
+                       const double* fy, const double* fz, const double* area, double* forcex,
+                       double* forcey, double* forcez) {
+  return spread_dev(c, false, ncap, cap_off, npts, x, y, z, fx, fy, fz, area, forcex, forcey,
+                    forcez);
+}
+
+int ibx_spread_cc_f64_dev(ibx_ctx* c, int64_t ncap, const int64_t* cap_off, int64_t npts,
+                          const double* x, const double* y, const double* z, const double* xn,
+                          const double* yn, const double* zn, const double* area, double* gx,
+                          double* gy, double* gz) {
+  return spread_dev(c, true, ncap, cap_off, npts, x, y, z, xn, yn, zn, area, gx, gy, gz);
+}
+
+// ---- host-array entry points --------------------------------------------------------------
+
+int ibx_eulr2lagr(ibx_ctx* c, const double* u, const double* v, const double* w, int64_t ncap,
+                  const int64_t* cap_off, const double* x, const double* y, const double* z,
+                  double* ul, double* vl, double* wl) {
+  if (!c) return IBX_E_ARG;
+  int rc = check_offsets(c, ncap, cap_off);
+  if (rc) return rc;
+  long long npts = ncap ? cap_off[ncap] : 0;
+  if (!(u && v && w)) return fail(c, IBX_E_ARG, "null field");
+  if (npts > 0 && !(x && y && z && ul && vl && wl)) return fail(c, IBX_E_ARG, "null pointer");
+  HIPCHK(c, hipSetDevice(c->device));
+  size_t fb = field_elems(c->grid) * sizeof(double), pb = (size_t)npts * sizeof(double);
+  if ((rc = upload(c, c->s_f1, u, fb)) || (rc = upload(c, c->s_f2, v, fb)) ||
+      (rc = upload(c, c->s_f3, w, fb)) || (rc = upload(c, c->s_x, x, pb)) ||
+      (rc = upload(c, c->s_y, y, pb)) || (rc = upload(c, c->s_z, z, pb)) ||
+      (rc = ensure(c, c->s_a1, pb)) || (rc = ensure(c, c->s_a2, pb)) ||
+      (rc = ensure(c, c->s_a3, pb)))
+    return rc;
+  if ((rc = enqueue_interp(c, (double*)c->s_f1.p, (double*)c->s_f2.p, (double*)c->s_f3.p, npts,
+                           (double*)c->s_x.p, (double*)c->s_y.p, (double*)c->s_z.p,
+                           (double*)c->s_a1.p, (double*)c->s_a2.p, (double*)c->s_a3.p)))
+    return rc;
+  if ((rc = download(c, ul, c->s_a1, pb)) || (rc = download(c, vl, c->s_a2, pb)) ||
+      (rc = download(c, wl, c->s_a3, pb)))
+    return rc;
+  return collect_errors(c);
+}
+
+static int spread_host(ibx_ctx* c, bool cc, int64_t ncap, const int64_t* cap_off,
+                       const double* x, const double* y, const double* z, const double* f1,
+                       const double* f2, const double* f3, const double* area, double* o1,
+                       double* o2, double* o3) {
+  if (!c) return IBX_E_ARG;
+  int rc = check_offsets(c, ncap, cap_off);
+  if (rc) return rc;
+  long long npts = ncap ? cap_off[ncap] : 0;
+  if (!(o1 && o2 && o3)) return fail(c, IBX_E_ARG, "null output field");
+  if (npts > 0 && !(x && y && z && f1 && f2 && f3 && area))
+    return fail(c, IBX_E_ARG, "null pointer");
+  HIPCHK(c, hipSetDevice(c->device));
+  size_t fb = field_elems(c->grid) * sizeof(double), pb = (size_t)npts * sizeof(double);
+  if ((rc = upload(c, c->s_x, x, pb)) || (rc = upload(c, c->s_y, y, pb)) ||
+      (rc = upload(c, c->s_z, z, pb)) || (rc = upload(c, c->s_a1, f1, pb)) ||
+      (rc = upload(c, c->s_a2, f2, pb)) || (rc = upload(c, c->s_a3, f3, pb)) ||
+      (rc = upload(c, c->s_a4, area, pb)) ||
+      (rc = upload(c, c->s_off, cap_off, (size_t)(ncap + 1) * sizeof(int64_t))) ||
+      (rc = ensure(c, c->s_o1, fb)) || (rc = ensure(c, c->s_o2, fb)) ||
+      (rc = ensure(c, c->s_o3, fb)))
+    return rc;
+  if ((rc = enqueue_spread(c, c->cfg, cc, ncap, (const long long*)c->s_off.p, npts,
+                           (double*)c->s_x.p, (double*)c->s_y.p, (double*)c->s_z.p,
+                           (double*)c->s_a1.p, (double*)c->s_a2.p, (double*)c->s_a3.p,
+                           (double*)c->s_a4.p, (double*)c->s_o1.p, (double*)c->s_o2.p,
+                           (double*)c->s_o3.p, true)))
+    return rc;
+  if ((rc = download(c, o1, c->s_o1, fb)) || (rc = download(c, o2, c->s_o2, fb)) ||
+      (rc = download(c, o3, c->s_o3, fb)))
+    return rc;
+  return collect_errors(c);
+}
+
+int ibx_lagr2eulr(ibx_ctx* c, int64_t ncap, const int64_t* cap_off, const double* x,
+                  const double* y, const double* z, const double* fx, const double* fy,
+                  const double* fz, const double* area, double* forcex, double* forcey,
+                  double* forcez) {
+  return spread_host(c, false, ncap, cap_off, x, y, z, fx, fy, fz, area, forcex, forcey,
+                     forcez);
+}
+
+int ibx_lagr2eulrind(ibx_ctx* c, int64_t ncap, const int64_t* cap_off, const double* x,
+                     const double* y, const double* z, const double* xn, const double* yn,
+                     const double* zn, const double* area, double* gx, double* gy,
+                     double* gz) {
+  return spread_host(c, true, ncap, cap_off, x, y, z, xn, yn, zn, area, gx, gy, gz);
+}
+
+// ---- Update_Pos --------------------------------------------------------------------------
+
+int ibx_update_pos_dev(ibx_ctx* c, int64_t ncap, const int64_t* cap_off, int64_t npts,
+                       double dt, double* x, double* y, double* z, double* xold, double* yold,
+                       const double* ul, const double* vl, const double* wl, double* centroid,
+                       double* umean) {
+  if (!c) return IBX_E_ARG;
+  if (ncap <= 0 || npts < 0) return fail(c, IBX_E_ARG, "bad sizes");
+  if (!(cap_off && x && y && z && ul && vl && wl && centroid && umean))
+    return fail(c, IBX_E_ARG, "null pointer");
+  if ((xold == nullptr) != (yold == nullptr))
+    return fail(c, IBX_E_ARG, "xold and yold must both be given or both be NULL");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = ensure(c, c->s_a4, (size_t)(2 * ncap));
+  if (rc) return rc;
+  signed char* sh = (signed char*)c->s_a4.p;
+  k_advance<<<blocks_for(npts, 256), 256, 0, c->stream>>>(c->gd, c->grid.lz, dt, npts, x, y, z,
+                                                          xold, yold, ul, vl, wl);
+  k_centroid<<<blocks_for(ncap, 64), 64, 0, c->stream>>>(c->gd, ncap, (const long long*)cap_off,
+                                                         x, y, z, ul, vl, wl, centroid, umean,
+                                                         sh);
+  k_shift<<<blocks_for(npts, 256), 256, 0, c->stream>>>(c->gd, ncap, (const long long*)cap_off,
+                                                        npts, sh, x, y);
+  HIPCHK(c, hipGetLastError());
+  return IBX_OK;
+}
+
+int ibx_update_pos(ibx_ctx* c, int64_t ncap, const int64_t* cap_off, double dt, double* x,
+                   double* y, double* z, double* xold, double* yold, const double* ul,
+                   const double* vl, const double* wl, double* centroid, double* umean,
+                   int64_t* nclamp) {
+  if (!c) return IBX_E_ARG;
+  int rc = check_offsets(c, ncap, cap_off);
+  if (rc) return rc;
+  if (ncap == 0) return IBX_OK;
+  long long npts = cap_off[ncap];
+  if ((xold == nullptr) != (yold == nullptr))
+    return fail(c, IBX_E_ARG, "xold and yold must both be given or both be NULL");
+  // points below / above the walls before the clamp (Update_Pos.f90:118-126 prints these)
+  if (nclamp) {
+    long long nc = 0;
+    for (long long q = 0; q < npts; ++q) {
+      double zn = z[q] + (wl[q] * dt);
+      if (zn < 0.0 || zn > c->grid.lz) ++nc;
+    }
+    *nclamp = nc;
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  size_t pb = (size_t)npts * sizeof(double), cb = (size_t)ncap * 3 * sizeof(double);
+  if ((rc = upload(c, c->s_x, x, pb)) || (rc = upload(c, c->s_y, y, pb)) ||
+      (rc = upload(c, c->s_z, z, pb)) || (rc = upload(c, c->s_a1, ul, pb)) ||
+      (rc = upload(c, c->s_a2, vl, pb)) || (rc = upload(c, c->s_a3, wl, pb)) ||
+      (rc = upload(c, c->s_off, cap_off, (size_t)(ncap + 1) * sizeof(int64_t))) ||
+      (rc = ensure(c, c->s_o1, cb)) || (rc = ensure(c, c->s_o2, cb)))
+    return rc;
+  if (xold) {
+    if ((rc = upload(c, c->s_f1, xold, pb)) || (rc = upload(c, c->s_f2, yold, pb))) return rc;
+  }
+  if ((rc = ibx_update_pos_dev(c, ncap, (const int64_t*)c->s_off.p, npts, dt, (double*)c->s_x.p,
+                               (double*)c->s_y.p, (double*)c->s_z.p,
+                               xold ? (double*)c->s_f1.p : nullptr,
+                               yold ? (double*)c->s_f2.p : nullptr, (double*)c->s_a1.p,
+                               (double*)c->s_a2.p, (double*)c->s_a3.p, (double*)c->s_o1.p,
+                               (double*)c->s_o2.p)))
+    return rc;
+  if ((rc = download(c, x, c->s_x, pb)) || (rc = download(c, y, c->s_y, pb)) ||
+      (rc = download(c, z, c->s_z, pb)) || (rc = download(c, centroid, c->s_o1, cb)) ||
+      (rc = download(c, umean, c->s_o2, cb)))
+    return rc;
+  if (xold) {
+    if ((rc = download(c, xold, c->s_f1, pb)) || (rc = download(c, yold, c->s_f2, pb)))
+      return rc;
+  }
+  return collect_errors(c);
+}
+
+// ---- batched / graph path -----------------------------------------------------------------
+
+int ibx_batch_num_configs(void) { return kNumCfgs; }
+
+const char* ibx_batch_config_name(int i) {
+  return (i >= 0 && i < kNumCfgs) ? kCfgs[i].name : "invalid";
+}
+
+int ibx_batch_create(ibx_ctx* c, ibx_batch** out, int64_t ncap, const int64_t* cap_off_host) {
+  if (!c || !out) return IBX_E_ARG;
+  *out = nullptr;
+  int rc = check_offsets(c, ncap, cap_off_host);
+  if (rc) return rc;
+  if (ncap <= 0) return fail(c, IBX_E_ARG, "a batch needs at least one capsule");
+  HIPCHK(c, hipSetDevice(c->device));
+  ibx_batch* b = new ibx_batch();
+  b->ctx = c;
+  b->ncap = ncap;
+  b->npts = cap_off_host[ncap];
+  b->cfg = c->cfg;
+  if ((rc = upload(c, b->d_off, cap_off_host, (size_t)(ncap + 1) * sizeof(int64_t)))) {
+    delete b;
+    return rc;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  *out = b;
+  return IBX_OK;
+}
+
+int ibx_batch_bind(ibx_batch* b, const ibx_batch_bufs* bufs) {
+  if (!b || !bufs) return IBX_E_ARG;
+  const ibx_batch_bufs& q = *bufs;
+  if (!(q.u && q.v && q.w && q.x && q.y && q.z && q.fx && q.fy && q.fz && q.area && q.ul &&
+        q.vl && q.wl && q.forcex && q.forcey && q.forcez))
+    return fail(b->ctx, IBX_E_ARG, "null buffer in ibx_batch_bufs");
+  b->bufs = q;
+  b->bound = true;
+  b->captured = false;
+  return IBX_OK;
+}
+
+static int batch_enqueue(ibx_batch* b, bool size_entries) {
+  ibx_ctx* c = b->ctx;
+  const ibx_batch_bufs& q = b->bufs;
+  int rc = enqueue_interp(c, q.u, q.v, q.w, b->npts, q.x, q.y, q.z, q.ul, q.vl, q.wl);
+  if (rc) return rc;
+  return enqueue_spread(c, b->cfg, false, b->ncap, (const long long*)b->d_off.p, b->npts, q.x,
+                        q.y, q.z, q.fx, q.fy, q.fz, q.area, q.forcex, q.forcey, q.forcez,
+                        size_entries);
+}
+
+static void drop_graph(ibx_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->captured = false;
+}
+
+int ibx_batch_set_config(ibx_batch* b, int choice) {
+  if (!b || choice < 0 || choice >= kNumCfgs) return IBX_E_ARG;
+  b->cfg = choice;
+  drop_graph(b);
+  return IBX_OK;
+}
+
+int ibx_batch_capture(ibx_batch* b, int use_graph) {
+  if (!b) return IBX_E_ARG;
+  ibx_ctx* c = b->ctx;
+  if (!b->bound) return fail(c, IBX_E_STATE, "batch not bound");
+  HIPCHK(c, hipSetDevice(c->device));
+  drop_graph(b);
+  // one eager pass sizes every work buffer (and the candidate list, with headroom)
+  int rc = batch_enqueue(b, true);
+  if (rc) return rc;
+  if ((rc = collect_errors(c))) return rc;
+  b->use_graph = use_graph != 0;
+  if (b->use_graph) {
+    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = batch_enqueue(b, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(c, IBX_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+    b->graph = g;
+    HIPCHK(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  b->captured = true;
+  return IBX_OK;
+}
+
+int ibx_batch_run(ibx_batch* b) {
+  if (!b) return IBX_E_ARG;
+  ibx_ctx* c = b->ctx;
+  if (!b->captured) return fail(c, IBX_E_STATE, "batch not captured");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (b->use_graph) {
+    HIPCHK(c, hipGraphLaunch(b->exec, c->stream));
+    return IBX_OK;
+  }
+  return batch_enqueue(b, false);
+}
+
+int ibx_batch_profile(ibx_batch* b, int reps, double* us_interp, double* us_bin,
+                      double* us_spread, double* us_total) {
+  // HIP events on the context stream bracket: interp | binning (count, scan, fill) |
+  // tile spread kernel | wall-plane zeroing.  us_spread is the tile kernel alone.
+  if (!b || reps <= 0) return IBX_E_ARG;
+  ibx_ctx* c = b->ctx;
+  if (!b->captured) return fail(c, IBX_E_STATE, "batch not captured");
+  HIPCHK(c, hipSetDevice(c->device));
+  const ibx_batch_bufs& q = b->bufs;
+  const TileCfg& cf = kCfgs[b->cfg];
+  TileGeom tg = geom(c->grid, cf, false);
+  hipEvent_t ev[5];
+  for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+  double t_i = 0, t_b = 0, t_s = 0, t_t = 0;
+  unsigned* cnt = (unsigned*)c->tcount.p;
+  unsigned* off = (unsigned*)c->toff.p;
+  unsigned long long* ent = (unsigned long long*)c->entries.p;
+  const long long* doff = (const long long*)b->d_off.p;
+  for (int r = 0; r < reps; ++r) {
+    HIPCHK(c, hipEventRecord(ev[0], c->stream));
+    int rc = enqueue_interp(c, q.u, q.v, q.w, b->npts, q.x, q.y, q.z, q.ul, q.vl, q.wl);
+    if (rc) return rc;
+    HIPCHK(c, hipEventRecord(ev[1], c->stream));
+    k_fill_u32<<<blocks_for(tg.ntiles, 256), 256, 0, c->stream>>>(cnt, tg.ntiles, 0u);
+    k_bin<0><<<blocks_for(b->npts, 256), 256, 0, c->stream>>>(c->gd, tg, false, b->ncap, doff,
+                                                              b->npts, q.x, q.y, q.z, cnt, off,
+                                                              ent, c->ecap, c->d_err);
+    if ((rc = enqueue_scan(c, cnt, tg.ntiles, off))) return rc;
+    k_fill_u32<<<blocks_for(tg.ntiles, 256), 256, 0, c->stream>>>(cnt, tg.ntiles, 0u);
+    k_bin<1><<<blocks_for(b->npts, 256), 256, 0, c->stream>>>(c->gd, tg, false, b->ncap, doff,
+                                                              b->npts, q.x, q.y, q.z, cnt, off,
+                                                              ent, c->ecap, c->d_err);
+    HIPCHK(c, hipEventRecord(ev[2], c->stream));
+    unsigned nblk;
+    if (cf.persist) {
+      long long want = (long long)c->num_cus * 4;
+      nblk = (unsigned)std::max<long long>(1, std::min<long long>(want, tg.ntiles));
+    } else {
+      nblk = (unsigned)(((tg.ntiles + 7) / 8) * 8);
+    }
+    cf.plain<<<nblk, SP_THREADS, 0, c->stream>>>(c->gd, tg, off, ent, c->ecap, q.x, q.y, q.z,
+                                                  q.fx, q.fy, q.fz, q.area, q.forcex, q.forcey,
+                                                  q.forcez);
+    HIPCHK(c, hipEventRecord(ev[3], c->stream));
+    k_zero_kplanes<<<blocks_for(2 * c->gd.sz, 256), 256, 0, c->stream>>>(c->gd, q.forcex,
+                                                                         q.forcey, q.forcez);
+    HIPCHK(c, hipEventRecord(ev[4], c->stream));
+    HIPCHK(c, hipEventSynchronize(ev[4]));
+    float a, bb, s, t;
+    HIPCHK(c, hipEventElapsedTime(&a, ev[0], ev[1]));
+    HIPCHK(c, hipEventElapsedTime(&bb, ev[1], ev[2]));
+    HIPCHK(c, hipEventElapsedTime(&s, ev[2], ev[3]));
+    HIPCHK(c, hipEventElapsedTime(&t, ev[0], ev[4]));
+    t_i += a, t_b += bb, t_s += s, t_t += t;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (us_interp) *us_interp = 1000.0 * t_i / reps;
+  if (us_bin) *us_bin = 1000.0 * t_b / reps;
+  if (us_spread) *us_spread = 1000.0 * t_s / reps;
+  if (us_total) *us_total = 1000.0 * t_t / reps;
+  return collect_errors(c);
+}
+
+int ibx_batch_autotune(ibx_batch* b, const char* cache_path, int* choice) {
+  if (!b) return IBX_E_ARG;
+  ibx_ctx* c = b->ctx;
+  if (!b->bound) return fail(c, IBX_E_STATE, "batch not bound");
+  HIPCHK(c, hipSetDevice(c->device));
+  char key[256];
+  snprintf(key, sizeof key, "grid=%dx%dx%d npts=%lld ncap=%lld", c->grid.itot, c->grid.jtot,
+           c->grid.ktot, b->npts, b->ncap);
+  if (cache_path) {
+    std::ifstream in(cache_path);
+    std::string line;
+    while (std::getline(in, line)) {
+      size_t bar = line.find(" | ");
+      if (bar == std::string::npos) continue;
+      if (line.substr(0, bar) == key) {
+        std::string name = line.substr(bar + 3);
+        for (int i = 0; i < kNumCfgs; ++i)
+          if (name == kCfgs[i].name) {
+            b->cfg = i;
+            drop_graph(b);
+            if (choice) *choice = i;
+            return IBX_OK;
+          }
+      }
+    }
+  }
+  int best = 0;
+  double best_t = 1e300;
+  for (int i = 0; i < kNumCfgs; ++i) {
+    b->cfg = i;
+    int rc = batch_enqueue(b, true);  // warm-up and sizing
+    if (rc) return rc;
+    if ((rc = collect_errors(c))) return rc;
+    double ti, tb, ts, tt;
+    b->captured = true;
+    if ((rc = ibx_batch_profile(b, 5, &ti, &tb, &ts, &tt))) return rc;
+    b->captured = false;
+    if (tt < best_t) {
+      best_t = tt;
+      best = i;
+    }
+  }
+  b->cfg = best;
+  drop_graph(b);
+  if (choice) *choice = best;
+  if (cache_path) {
+    std::ofstream outf(cache_path, std::ios::app);
+    if (outf) outf << key << " | " << kCfgs[best].name << "\n";
+  }
+  return IBX_OK;
+}
+
+void ibx_batch_destroy(ibx_batch* b) {
+  if (!b) return;
+  if (b->ctx) (void)hipSetDevice(b->ctx->device);
+  if (b->ctx && b->ctx->stream) (void)hipStreamSynchronize(b->ctx->stream);
+  drop_graph(b);
+  release(b->d_off);
+  delete b;
+}
+
+}  // extern "C"

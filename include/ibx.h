@@ -181,11 +181,13 @@ const char* ibx_batch_config_name(int choice);
  * (use_graph = 0) on every ibx_batch_run. */
 int ibx_batch_capture(ibx_batch* b, int use_graph);
 int ibx_batch_run(ibx_batch* b);
-/* Run interp + spread `reps` times eagerly, with HIP events on the context stream around each
- * stage, and return average durations in microseconds: the interpolation kernel, the binning
- * (count, scan, fill), the tile spread kernel alone, and the whole sequence. */
-int ibx_batch_profile(ibx_batch* b, int reps, double* us_interp, double* us_bin,
-                      double* us_spread, double* us_total);
+/* Run interp + spread `reps` times eagerly with HIP events on the context stream around each
+ * stage; us[0..IBX_PROF_N) receives, in microseconds: [0] interpolation kernel, [1] binning
+ * (count, scan, fill), [2] tile spread kernel, [3] whole sequence, [4] tile spread kernel's
+ * non-empty tiles and [5] its zero rows when run as two launches; and the work counts of the
+ * last run: [6] non-empty tiles, [7] tiles, [8] candidate entries. */
+#define IBX_PROF_N 9
+int ibx_batch_profile(ibx_batch* b, int reps, double* us);
 void ibx_batch_destroy(ibx_batch* b);
 
 #ifdef __cplusplus

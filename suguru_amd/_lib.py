@@ -102,9 +102,7 @@ def load() -> ctypes.CDLL:
         "ibx_batch_config_name": (ctypes.c_char_p, [ctypes.c_int]),
         "ibx_batch_capture": (ctypes.c_int, [P, ctypes.c_int]),
         "ibx_batch_run": (ctypes.c_int, [P]),
-        "ibx_batch_profile": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(D),
-                                             ctypes.POINTER(D), ctypes.POINTER(D),
-                                             ctypes.POINTER(D)]),
+        "ibx_batch_profile": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(D)]),
         "ibx_batch_destroy": (None, [P]),
     }
     for name, (res, args) in sig.items():

@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -29,31 +30,31 @@ struct DevBuf {
 };
 
 // ---- tile configurations (compile-time instances; the batch autotuner picks one) --------
-using SpreadFn = void (*)(GridD, TileGeom, const unsigned*, unsigned long long*,
+using SpreadFn = void (*)(GridD, TileGeom, const unsigned*, int*, int, unsigned long long*,
                           unsigned long long, const double*, const double*, const double*,
                           const double*, const double*, const double*, const double*, double*,
                           double*, double*);
 
 struct TileCfg {
-  int ti, tj, tk;
-  bool persist;
+  int tj, tk;     // rows per tile (each row: one aligned 16-double chunk)
+  int wg_per_cu;  // workgroups launched per compute unit
   const char* name;
   SpreadFn plain;
   SpreadFn cc;
 };
 
-#define IBX_CFG(TI, TJ, TK, P, NAME) \
-  {TI, TJ, TK, P, NAME, &k_spread<TI, TJ, TK, false, P>, &k_spread<TI, TJ, TK, true, P>}
+#define IBX_CFG(TJ, TK, W, NAME) \
+  {TJ, TK, W, NAME, &k_spread<TJ, TK, false>, &k_spread<TJ, TK, true>}
 
 const TileCfg kCfgs[] = {
-    IBX_CFG(16, 8, 8, false, "tile16x8x8"),
-    IBX_CFG(8, 8, 8, false, "tile8x8x8"),
-    IBX_CFG(32, 8, 4, false, "tile32x8x4"),
-    IBX_CFG(16, 16, 4, false, "tile16x16x4"),
-    IBX_CFG(32, 4, 8, false, "tile32x4x8"),
-    IBX_CFG(16, 8, 8, true, "tile16x8x8-persistent"),
-    IBX_CFG(8, 8, 8, true, "tile8x8x8-persistent"),
-    IBX_CFG(32, 8, 4, true, "tile32x8x4-persistent"),
+    IBX_CFG(8, 8, 8, "chunk16x8x8"),
+    IBX_CFG(8, 4, 8, "chunk16x8x4"),
+    IBX_CFG(4, 8, 8, "chunk16x4x8"),
+    IBX_CFG(16, 4, 8, "chunk16x16x4"),
+    IBX_CFG(8, 8, 4, "chunk16x8x8-w4"),
+    IBX_CFG(8, 4, 12, "chunk16x8x4-w12"),
+    IBX_CFG(16, 8, 6, "chunk16x16x8"),
+    IBX_CFG(4, 16, 8, "chunk16x4x16"),
 };
 constexpr int kNumCfgs = (int)(sizeof(kCfgs) / sizeof(kCfgs[0]));
 
@@ -69,7 +70,7 @@ struct ibx_ctx {
   std::string err;
   int* d_err = nullptr;
   // spread binning
-  DevBuf tcount, toff, tpart, entries;
+  DevBuf tcount, toff, tpart, tlist, tmeta, entries;
   // staging for the host-array entry points
   DevBuf s_f1, s_f2, s_f3, s_x, s_y, s_z, s_a1, s_a2, s_a3, s_a4, s_o1, s_o2, s_o3, s_off;
   int cfg = 0;
@@ -131,12 +132,12 @@ size_t field_elems(const ibx_grid& g) {
 
 TileGeom geom(const ibx_grid& g, const TileCfg& cf, bool cc) {
   TileGeom t{};
-  t.ti = cf.ti, t.tj = cf.tj, t.tk = cf.tk;
+  t.ti = SP_CW, t.tj = cf.tj, t.tk = cf.tk;
   t.kbase = cc ? 0 : 1;
   t.nkp = cc ? g.ktot + 2 : g.ktot;
   t.klo = cc ? 0 : 1;
   t.khi = cc ? g.ktot + 1 : g.ktot;
-  t.nti = (g.itot + cf.ti - 1) / cf.ti;
+  t.nti = (g.itot + SP_CW) / SP_CW + 1;  // a row's cells reach position itot+1+15
   t.ntj = (g.jtot + cf.tj - 1) / cf.tj;
   t.ntk = (t.nkp + cf.tk - 1) / cf.tk;
   t.ntiles = (long long)t.nti * t.ntj * t.ntk;
@@ -156,26 +157,31 @@ int check_offsets(ibx_ctx* c, long long ncap, const int64_t* off) {
   return IBX_OK;
 }
 
-// exclusive scan counts[0..n) -> off[0..n]
+// exclusive scan counts[0..n) -> off[0..n]; non-empty tiles -> tlist, their number -> tmeta[0]
 int enqueue_scan(ibx_ctx* c, const unsigned* counts, long long n, unsigned* off) {
   long long nb = (n + SCAN_B - 1) / SCAN_B;
-  int rc = ensure(c, c->tpart, (size_t)std::max<long long>(nb, 1) * sizeof(unsigned));
+  int rc = ensure(c, c->tpart, (size_t)std::max<long long>(nb, 1) * sizeof(unsigned long long));
   if (rc) return rc;
-  unsigned* part = (unsigned*)c->tpart.p;
+  if ((rc = ensure(c, c->tlist, (size_t)n * sizeof(int)))) return rc;
+  if ((rc = ensure(c, c->tmeta, 9 * SP_QSTRIDE * sizeof(int)))) return rc;
+  unsigned long long* part = (unsigned long long*)c->tpart.p;
   k_scan_reduce<<<(unsigned)nb, 256, 0, c->stream>>>(counts, n, part);
   k_scan_parts<<<1, 256, 0, c->stream>>>(part, nb);
-  k_scan_apply<<<(unsigned)nb, 256, 0, c->stream>>>(counts, n, part, off);
+  k_scan_apply<<<(unsigned)nb, 256, 0, c->stream>>>(counts, n, part, off, (int*)c->tlist.p,
+                                                    (int*)c->tmeta.p);
   HIPCHK(c, hipGetLastError());
   return IBX_OK;
 }
 
-// The whole spread (binning, scan, fill, tile kernel, wall planes) on the context stream.
+// The whole spread (binning, scan + compaction, fill, the two-queue tile kernel) on the
+// context stream.  ev_tile (optional) is recorded just before the tile kernel.
 // size_entries: read the candidate count back and grow the list (eager path); otherwise
 // the current capacity is used and overflow is reported through the device error word.
 int enqueue_spread(ibx_ctx* c, int cfgid, bool cc, long long ncap, const long long* d_off,
                    long long npts, const double* x, const double* y, const double* z,
                    const double* f1, const double* f2, const double* f3, const double* area,
-                   double* o1, double* o2, double* o3, bool size_entries) {
+                   double* o1, double* o2, double* o3, bool size_entries,
+                   hipEvent_t ev_tile = nullptr, hipEvent_t ev_split = nullptr) {
   const TileCfg& cf = kCfgs[cfgid];
   TileGeom tg = geom(c->grid, cf, cc);
   int rc;
@@ -190,7 +196,7 @@ int enqueue_spread(ibx_ctx* c, int cfgid, bool cc, long long ncap, const long lo
   k_fill_u32<<<blocks_for(tg.ntiles, 256), 256, 0, c->stream>>>(cnt, tg.ntiles, 0u);
   if (npts > 0)
     k_bin<0><<<blocks_for(npts, 256), 256, 0, c->stream>>>(
-        c->gd, tg, cc, ncap, d_off, npts, x, y, z, cnt, off, (unsigned long long*)c->entries.p,
+        c->gd, tg, ncap, d_off, npts, x, y, z, cnt, off, (unsigned long long*)c->entries.p,
         c->ecap, c->d_err);
   if ((rc = enqueue_scan(c, cnt, tg.ntiles, off))) return rc;
   if (size_entries) {
@@ -207,22 +213,35 @@ int enqueue_spread(ibx_ctx* c, int cfgid, bool cc, long long ncap, const long lo
   k_fill_u32<<<blocks_for(tg.ntiles, 256), 256, 0, c->stream>>>(cnt, tg.ntiles, 0u);
   if (npts > 0)
     k_bin<1><<<blocks_for(npts, 256), 256, 0, c->stream>>>(
-        c->gd, tg, cc, ncap, d_off, npts, x, y, z, cnt, off, (unsigned long long*)c->entries.p,
+        c->gd, tg, ncap, d_off, npts, x, y, z, cnt, off, (unsigned long long*)c->entries.p,
         c->ecap, c->d_err);
+  if (ev_tile) HIPCHK(c, hipEventRecord(ev_tile, c->stream));
   SpreadFn fn = cc ? cf.cc : cf.plain;
-  unsigned nblk;
-  if (cf.persist) {
-    long long want = (long long)c->num_cus * 4;
-    nblk = (unsigned)std::max<long long>(1, std::min<long long>(want, tg.ntiles));
-  } else {
-    long long per = (tg.ntiles + 7) / 8;
-    nblk = (unsigned)(per * 8);
-  }
-  fn<<<nblk, SP_THREADS, 0, c->stream>>>(c->gd, tg, off, (unsigned long long*)c->entries.p,
-                                          c->ecap, x, y, z, f1, f2, f3, area, o1, o2, o3);
+  const unsigned nblk = (unsigned)((long long)c->num_cus * cf.wg_per_cu);
   if (!cc) {
-    long long n2 = 2 * c->gd.sz;
-    k_zero_kplanes<<<blocks_for(n2, 256), 256, 0, c->stream>>>(c->gd, o1, o2, o3);
+    const long long n2 = 2 * c->gd.sz;
+    k_zero_kplanes<<<blocks_for((n2 + 1) / 2, 256), 256, 0, c->stream>>>(c->gd, o1, o2, o3);
+  }
+  if (!ev_split) {
+    fn<<<nblk, SP_THREADS, 0, c->stream>>>(c->gd, tg, off, (int*)c->tmeta.p, 3,
+                                            (unsigned long long*)c->entries.p, c->ecap, x, y, z,
+                                            f1, f2, f3, area, o1, o2, o3);
+  } else {
+    // the same work as two launches: tiles with candidates, then tiles without (timing)
+    fn<<<nblk, SP_THREADS, 0, c->stream>>>(c->gd, tg, off, (int*)c->tmeta.p, 1,
+                                            (unsigned long long*)c->entries.p, c->ecap, x, y, z,
+                                            f1, f2, f3, area, o1, o2, o3);
+    HIPCHK(c, hipEventRecord(ev_split, c->stream));
+    for (int q = 1; q <= 8; ++q)
+      k_fill_i32<<<1, 64, 0, c->stream>>>((int*)c->tmeta.p + SP_QSTRIDE * q, 1, 0);
+    fn<<<nblk, SP_THREADS, 0, c->stream>>>(c->gd, tg, off, (int*)c->tmeta.p, 2,
+                                            (unsigned long long*)c->entries.p, c->ecap, x, y, z,
+                                            f1, f2, f3, area, o1, o2, o3);
+  }
+  {
+    const int kf = tg.kbase, kl = tg.kbase + tg.nkp - 1;
+    const long long nj = (long long)(kl - kf + 1) * 2 * c->gd.sy;
+    k_jhalo<<<blocks_for(nj, 256), 256, 0, c->stream>>>(c->gd, kf, kl, o1, o2, o3);
   }
   HIPCHK(c, hipGetLastError());
   return IBX_OK;
@@ -232,8 +251,8 @@ int enqueue_interp(ibx_ctx* c, const double* u, const double* v, const double* w
                    long long npts, const double* x, const double* y, const double* z,
                    double* ul, double* vl, double* wl) {
   if (npts > 0)
-    k_interp<<<blocks_for(npts, 256), 256, 0, c->stream>>>(c->gd, u, v, w, npts, x, y, z, ul,
-                                                            vl, wl, c->d_err);
+    k_interp<<<dim3(blocks_for(npts, 256), 3), 256, 0, c->stream>>>(c->gd, u, v, w, npts, x, y,
+                                                                      z, ul, vl, wl, c->d_err);
   HIPCHK(c, hipGetLastError());
   return IBX_OK;
 }
@@ -326,7 +345,7 @@ void ibx_destroy(ibx_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  DevBuf* all[] = {&c->tcount, &c->toff, &c->tpart, &c->entries, &c->s_f1, &c->s_f2,
+  DevBuf* all[] = {&c->tcount, &c->toff, &c->tpart, &c->tlist, &c->tmeta, &c->entries, &c->s_f1, &c->s_f2,
                    &c->s_f3,   &c->s_x,  &c->s_y,   &c->s_z,     &c->s_a1, &c->s_a2,
                    &c->s_a3,   &c->s_a4, &c->s_o1,  &c->s_o2,    &c->s_o3, &c->s_off};
   for (DevBuf* b : all) release(*b);
@@ -364,6 +383,12 @@ int ibx_interp_f64_dev(ibx_ctx* c, const double* u, const double* v, const doubl
   return enqueue_interp(c, u, v, w, npts, x, y, z, ul, vl, wl);
 }
 
+// The spread writes 16-byte pairs of cells; fields allocated by hipMalloc (or torch) are
+// 256-byte aligned, and a 128-byte aligned field gets whole-cache-line stores.
+static bool aligned16(const double* a, const double* b, const double* c) {
+  return (((uintptr_t)a | (uintptr_t)b | (uintptr_t)c) & 15) == 0;
+}
+
 static int spread_dev(ibx_ctx* c, bool cc, int64_t ncap, const int64_t* cap_off, int64_t npts,
                       const double* x, const double* y, const double* z, const double* f1,
                       const double* f2, const double* f3, const double* area, double* o1,
@@ -372,6 +397,7 @@ static int spread_dev(ibx_ctx* c, bool cc, int64_t ncap, const int64_t* cap_off,
   if (ncap < 0 || npts < 0) return fail(c, IBX_E_ARG, "negative size");
   if (npts >= (1ll << 28)) return fail(c, IBX_E_ARG, "more than 2^28 - 1 points");
   if (!(o1 && o2 && o3)) return fail(c, IBX_E_ARG, "null output field");
+  if (!aligned16(o1, o2, o3)) return fail(c, IBX_E_ARG, "output fields not 16-byte aligned");
   if (npts > 0 && !(cap_off && x && y && z && f1 && f2 && f3 && area && ncap > 0))
     return fail(c, IBX_E_ARG, "null pointer or no capsules");
   HIPCHK(c, hipSetDevice(c->device));
@@ -582,6 +608,8 @@ int ibx_batch_bind(ibx_batch* b, const ibx_batch_bufs* bufs) {
   if (!(q.u && q.v && q.w && q.x && q.y && q.z && q.fx && q.fy && q.fz && q.area && q.ul &&
         q.vl && q.wl && q.forcex && q.forcey && q.forcez))
     return fail(b->ctx, IBX_E_ARG, "null buffer in ibx_batch_bufs");
+  if (!aligned16(q.forcex, q.forcey, q.forcez))
+    return fail(b->ctx, IBX_E_ARG, "force fields not 16-byte aligned");
   b->bufs = q;
   b->bound = true;
   b->captured = false;
@@ -619,10 +647,24 @@ int ibx_batch_capture(ibx_batch* b, int use_graph) {
   if (!b->bound) return fail(c, IBX_E_STATE, "batch not bound");
   HIPCHK(c, hipSetDevice(c->device));
   drop_graph(b);
-  // one eager pass sizes every work buffer (and the candidate list, with headroom)
+  // one eager pass sizes every work buffer; the candidate list then gets room for the points
+  // to move: 1.5x the measured count, and at least 4 entries per point (a point's window
+  // lists it in ~2-3 tiles on average)
   int rc = batch_enqueue(b, true);
   if (rc) return rc;
   if ((rc = collect_errors(c))) return rc;
+  {
+    TileGeom tg = geom(c->grid, kCfgs[b->cfg], false);
+    unsigned total = 0;
+    HIPCHK(c, hipMemcpyAsync(&total, (unsigned*)c->toff.p + tg.ntiles, sizeof(unsigned),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    size_t want = std::max<size_t>((size_t)total + total / 2, (size_t)(4 * b->npts)) + 4096;
+    if (want > c->ecap) {
+      if ((rc = ensure(c, c->entries, want * sizeof(unsigned long long)))) return rc;
+      c->ecap = c->entries.bytes / sizeof(unsigned long long);
+    }
+  }
   b->use_graph = use_graph != 0;
   if (b->use_graph) {
     HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -653,66 +695,55 @@ int ibx_batch_run(ibx_batch* b) {
   return batch_enqueue(b, false);
 }
 
-int ibx_batch_profile(ibx_batch* b, int reps, double* us_interp, double* us_bin,
-                      double* us_spread, double* us_total) {
-  // HIP events on the context stream bracket: interp | binning (count, scan, fill) |
-  // tile spread kernel | wall-plane zeroing.  us_spread is the tile kernel alone.
-  if (!b || reps <= 0) return IBX_E_ARG;
+int ibx_batch_profile(ibx_batch* b, int reps, double* us) {
+  // HIP events on the context stream bracket each stage of `reps` eager runs.
+  if (!b || reps <= 0 || !us) return IBX_E_ARG;
   ibx_ctx* c = b->ctx;
   if (!b->captured) return fail(c, IBX_E_STATE, "batch not captured");
   HIPCHK(c, hipSetDevice(c->device));
   const ibx_batch_bufs& q = b->bufs;
-  const TileCfg& cf = kCfgs[b->cfg];
-  TileGeom tg = geom(c->grid, cf, false);
-  hipEvent_t ev[5];
+  hipEvent_t ev[6];
   for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
-  double t_i = 0, t_b = 0, t_s = 0, t_t = 0;
-  unsigned* cnt = (unsigned*)c->tcount.p;
-  unsigned* off = (unsigned*)c->toff.p;
-  unsigned long long* ent = (unsigned long long*)c->entries.p;
-  const long long* doff = (const long long*)b->d_off.p;
-  for (int r = 0; r < reps; ++r) {
-    HIPCHK(c, hipEventRecord(ev[0], c->stream));
-    int rc = enqueue_interp(c, q.u, q.v, q.w, b->npts, q.x, q.y, q.z, q.ul, q.vl, q.wl);
-    if (rc) return rc;
-    HIPCHK(c, hipEventRecord(ev[1], c->stream));
-    k_fill_u32<<<blocks_for(tg.ntiles, 256), 256, 0, c->stream>>>(cnt, tg.ntiles, 0u);
-    k_bin<0><<<blocks_for(b->npts, 256), 256, 0, c->stream>>>(c->gd, tg, false, b->ncap, doff,
-                                                              b->npts, q.x, q.y, q.z, cnt, off,
-                                                              ent, c->ecap, c->d_err);
-    if ((rc = enqueue_scan(c, cnt, tg.ntiles, off))) return rc;
-    k_fill_u32<<<blocks_for(tg.ntiles, 256), 256, 0, c->stream>>>(cnt, tg.ntiles, 0u);
-    k_bin<1><<<blocks_for(b->npts, 256), 256, 0, c->stream>>>(c->gd, tg, false, b->ncap, doff,
-                                                              b->npts, q.x, q.y, q.z, cnt, off,
-                                                              ent, c->ecap, c->d_err);
-    HIPCHK(c, hipEventRecord(ev[2], c->stream));
-    unsigned nblk;
-    if (cf.persist) {
-      long long want = (long long)c->num_cus * 4;
-      nblk = (unsigned)std::max<long long>(1, std::min<long long>(want, tg.ntiles));
-    } else {
-      nblk = (unsigned)(((tg.ntiles + 7) / 8) * 8);
+  double acc[IBX_PROF_N] = {0};
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int r = 0; r < reps; ++r) {
+      HIPCHK(c, hipEventRecord(ev[0], c->stream));
+      int rc = enqueue_interp(c, q.u, q.v, q.w, b->npts, q.x, q.y, q.z, q.ul, q.vl, q.wl);
+      if (rc) return rc;
+      HIPCHK(c, hipEventRecord(ev[1], c->stream));
+      rc = enqueue_spread(c, b->cfg, false, b->ncap, (const long long*)b->d_off.p, b->npts, q.x,
+                          q.y, q.z, q.fx, q.fy, q.fz, q.area, q.forcex, q.forcey, q.forcez,
+                          false, ev[2], pass == 1 ? ev[4] : nullptr);
+      if (rc) return rc;
+      HIPCHK(c, hipEventRecord(ev[3], c->stream));
+      HIPCHK(c, hipEventSynchronize(ev[3]));
+      float a, bb, s, t;
+      if (pass == 0) {
+        HIPCHK(c, hipEventElapsedTime(&a, ev[0], ev[1]));
+        HIPCHK(c, hipEventElapsedTime(&bb, ev[1], ev[2]));
+        HIPCHK(c, hipEventElapsedTime(&s, ev[2], ev[3]));
+        HIPCHK(c, hipEventElapsedTime(&t, ev[0], ev[3]));
+        acc[0] += a, acc[1] += bb, acc[2] += s, acc[3] += t;
+      } else {
+        HIPCHK(c, hipEventElapsedTime(&a, ev[2], ev[4]));
+        HIPCHK(c, hipEventElapsedTime(&bb, ev[4], ev[3]));
+        acc[4] += a, acc[5] += bb;
+      }
     }
-    cf.plain<<<nblk, SP_THREADS, 0, c->stream>>>(c->gd, tg, off, ent, c->ecap, q.x, q.y, q.z,
-                                                  q.fx, q.fy, q.fz, q.area, q.forcex, q.forcey,
-                                                  q.forcez);
-    HIPCHK(c, hipEventRecord(ev[3], c->stream));
-    k_zero_kplanes<<<blocks_for(2 * c->gd.sz, 256), 256, 0, c->stream>>>(c->gd, q.forcex,
-                                                                         q.forcey, q.forcez);
-    HIPCHK(c, hipEventRecord(ev[4], c->stream));
-    HIPCHK(c, hipEventSynchronize(ev[4]));
-    float a, bb, s, t;
-    HIPCHK(c, hipEventElapsedTime(&a, ev[0], ev[1]));
-    HIPCHK(c, hipEventElapsedTime(&bb, ev[1], ev[2]));
-    HIPCHK(c, hipEventElapsedTime(&s, ev[2], ev[3]));
-    HIPCHK(c, hipEventElapsedTime(&t, ev[0], ev[4]));
-    t_i += a, t_b += bb, t_s += s, t_t += t;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
-  if (us_interp) *us_interp = 1000.0 * t_i / reps;
-  if (us_bin) *us_bin = 1000.0 * t_b / reps;
-  if (us_spread) *us_spread = 1000.0 * t_s / reps;
-  if (us_total) *us_total = 1000.0 * t_t / reps;
+  for (int i = 0; i < 6; ++i) us[i] = 1000.0 * acc[i] / reps;
+  // work counts of the last run: non-empty tiles and candidate entries
+  int meta0 = 0;
+  unsigned tot = 0;
+  TileGeom tg = geom(c->grid, kCfgs[b->cfg], false);
+  HIPCHK(c, hipMemcpyAsync(&meta0, c->tmeta.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(&tot, (unsigned*)c->toff.p + tg.ntiles, sizeof(unsigned),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  us[6] = (double)meta0;
+  us[7] = (double)tg.ntiles;
+  us[8] = (double)tot;
   return collect_errors(c);
 }
 
@@ -749,12 +780,12 @@ int ibx_batch_autotune(ibx_batch* b, const char* cache_path, int* choice) {
     int rc = batch_enqueue(b, true);  // warm-up and sizing
     if (rc) return rc;
     if ((rc = collect_errors(c))) return rc;
-    double ti, tb, ts, tt;
     b->captured = true;
-    if ((rc = ibx_batch_profile(b, 5, &ti, &tb, &ts, &tt))) return rc;
+    double us[IBX_PROF_N];
+    if ((rc = ibx_batch_profile(b, 5, us))) return rc;
     b->captured = false;
-    if (tt < best_t) {
-      best_t = tt;
+    if (us[3] < best_t) {
+      best_t = us[3];
       best = i;
     }
   }

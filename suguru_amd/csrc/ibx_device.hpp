@@ -25,8 +25,8 @@ struct GridD {
 // k in [kbase, kbase+nkp-1] (kbase = 1, nkp = ktot for lagr2eulr; kbase = 0, nkp = ktot+2
 // for lagr2eulrind whose window reaches the wall halo planes, interp_spread.f90:1311-1316).
 struct TileGeom {
-  int ti, tj, tk;     // tile extent in cells
-  int nti, ntj, ntk;  // tiles per direction
+  int ti, tj, tk;     // tile extent: ti = chunk width (16 positions per row), tj x tk rows
+  int nti, ntj, ntk;  // tiles per direction (nti: chunk columns per row)
   int kbase, nkp;     // first k plane and number of k planes covered
   int klo, khi;       // k clamp of the stencil window
   long long ntiles;
@@ -34,48 +34,47 @@ struct TileGeom {
 
 enum : int { ERR_NONFINITE = 1, ERR_CAPACITY = 2 };
 
-// kernel.f90:7-25, in the reference's operation order.
+
+// kernel.f90:7-25, in the reference's operation order.  Both branches are evaluated and one
+// is selected, so a wavefront with mixed |r| runs a single square root; each branch performs
+// exactly the reference's operations, hence the same rounded result.
 __device__ __forceinline__ double phi(double r) {
-  double ar = fabs(r);
-  if (ar > 1.5) return 0.0;
-  if (ar > 0.5) {
-    double t = 1.0 - ar;
-    return (1.0 / 6.0) * ((5.0 - 3.0 * ar) - sqrt(-(3.0 * (t * t)) + 1.0));
-  }
-  return (1.0 / 3.0) * (1.0 + sqrt(-(3.0 * (r * r)) + 1.0));
+  const double ar = fabs(r);
+  const bool outer = ar > 0.5;
+  const double t = 1.0 - ar;
+  const double arg = outer ? (-(3.0 * (t * t)) + 1.0) : (-(3.0 * (r * r)) + 1.0);
+  const double sq = sqrt(arg);
+  const double v = outer ? (1.0 / 6.0) * ((5.0 - 3.0 * ar) - sq) : (1.0 / 3.0) * (1.0 + sq);
+  return ar > 1.5 ? 0.0 : v;
 }
 
 // Image slots of a master that is its own neighbour (interp_spread.f90:82-141):
-// slot -> shift in units of (lx, ly).
-__device__ __forceinline__ int slot_sx(int s) {
-  // s: 0  1  2  3  4  5  6  7  8
-  //    0 +1 +1  0 -1 -1 -1  0 +1
-  return (s == 1 || s == 2 || s == 8) ? 1 : ((s == 4 || s == 5 || s == 6) ? -1 : 0);
+// slot -> shift in units of (lx, ly), coded 0: none, 1: +, 2: -.
+//   s:  0  1  2  3  4  5  6  7  8
+//   x:  0  +  +  0  -  -  -  0  +
+//   y:  0  0  -  -  -  0  +  +  +
+__device__ __forceinline__ int slot_xs(int s) {
+  return (s == 1 || s == 2 || s == 8) ? 1 : ((s == 4 || s == 5 || s == 6) ? 2 : 0);
 }
-__device__ __forceinline__ int slot_sy(int s) {
-  //    0  0 -1 -1 -1  0 +1 +1 +1
-  return (s == 2 || s == 3 || s == 4) ? -1 : ((s == 6 || s == 7 || s == 8) ? 1 : 0);
+__device__ __forceinline__ int slot_ys(int s) {
+  return (s == 2 || s == 3 || s == 4) ? 2 : ((s == 6 || s == 7 || s == 8) ? 1 : 0);
 }
 // Loop index nb -> data slot (interp_spread.f90:333-336).
 __device__ __forceinline__ int nb_slot(int nb) { return nb == 0 ? 0 : (nb + 4 > 8 ? nb - 4 : nb + 4); }
 
-// The copy of a point in slot s: anb(s)%xfp = xfp then +-lx, anb(s)%yfp = yfp then +-ly.
-__device__ __forceinline__ void slot_coords(const GridD& g, int s, double x, double y, double& cx,
-                                            double& cy) {
-  int sx = slot_sx(s), sy = slot_sy(s);
-  double xs = x, ys = y;
-  if (sx > 0) xs = xs + g.lx;
-  if (sx < 0) xs = xs - g.lx;
-  if (sy > 0) ys = ys + g.ly;
-  if (sy < 0) ys = ys - g.ly;
-  cx = (xs - g.bl) * g.dxi;
-  cy = (ys - g.bf) * g.dyi;
+// Stencil coordinate of the copy shifted by code sh (0 none, 1 +L, 2 -L):
+// anb(s)%xfp = xfp (+-lx), coorxfp = (anb(s)%xfp - boundleftmyid)*dxi (:337-338).
+__device__ __forceinline__ double shifted_coord(double x, int sh, double L, double b, double inv) {
+  double xs = x;
+  if (sh == 1) xs = xs + L;
+  if (sh == 2) xs = xs - L;
+  return (xs - b) * inv;
 }
 
 // Support of the stencil along one direction.  With f = floor(c), every tap whose weight can
 // be non-zero -- |i-c| < 1.5 (centred) or |i-c-1/2| < 1.5 (staggered) -- lies in [f-1, f+2];
 // every tap of the reference's window nint(c-2)..nint(c+2) outside it has weight exactly 0
-// (its rounded distance is >= 1.5), and adding +-0 leaves a non-negative-zero sum unchanged.
+// (its rounded distance is >= 1.5), and adding +-0 leaves a sum that is not -0 unchanged.
 // Returns false if [f-1,f+2] misses [lo,hi].  `c` must be finite.
 __device__ __forceinline__ bool window(double c, int lo, int hi, int& w0, int& a, int& b) {
   if (!(c > (double)lo - 4.0 && c < (double)hi + 4.0)) return false;
@@ -86,27 +85,118 @@ __device__ __forceinline__ bool window(double c, int lo, int hi, int& w0, int& a
   return a <= b;
 }
 
-// Weights on the 4-tap support [w0, w0+3]: ns = kernel(i-c) (kernel*s in the reference),
-// st = kernel(i-c-1/2) (kernel* without s), zero outside [a,b].
-__device__ __forceinline__ void weights4(double c, int w0, int a, int b, double ns[4],
-                                         double st[4]) {
+// Weights of one copy of a point along one direction, on the 4-tap support [w0, w0+3] of
+// window(): ns = kernel(i-c) (the reference's kernel*s), st = kernel(i-c-1/2) (kernel*),
+// zero outside the clamped window [a,b].  The staggered weight of tap w0 is always 0
+// (distance <= -1.5), so st is kept for taps 1..3; the centred weights are non-zero on at
+// most three consecutive taps, {0,1,2} when tap 0's distance exceeds -1.5, else within
+// {1,2,3}, so ns is kept for taps na..na+2.  Only those six are evaluated.
+struct Axis {
+  int w0, na;
+  double ns[3];  // taps na, na+1, na+2
+  double st[3];  // taps 1, 2, 3
+};
+
+__device__ __forceinline__ bool axis_of(double c, int lo, int hi, Axis& ax) {
+  int a, b;
+  if (!window(c, lo, hi, ax.w0, a, b)) return false;
+  const double rs0 = (1.0 * (double)ax.w0) - c;
+  ax.na = (rs0 > -1.5) ? 0 : 1;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    int i = w0 + t;
-    double rs = (1.0 * (double)i) - c;
-    double r = rs - 0.5;
-    bool in = (i >= a) && (i <= b);
-    ns[t] = in ? phi(rs) : 0.0;
-    st[t] = in ? phi(r) : 0.0;
+  for (int t = 0; t < 3; ++t) {
+    const int in = ax.w0 + ax.na + t;
+    const double rn = (1.0 * (double)in) - c;
+    ax.ns[t] = (in >= a && in <= b) ? phi(rn) : 0.0;
+    const int is = ax.w0 + 1 + t;
+    const double rs = (1.0 * (double)is) - c;
+    const double r = rs - 0.5;
+    ax.st[t] = (is >= a && is <= b) ? phi(r) : 0.0;
   }
+  return true;
 }
 
 // ---------------------------------------------------------------------------------------
 // Interpolation: eulr2lagr (interp_spread.f90:315-383 and the owner sum :426-440).
-// One thread per Lagrangian point.  For every image slot whose clamped window meets the
-// grid, the partial sum runs k -> j -> i over the taps of non-zero weight (skipping a zero
-// term is exact), then the partials are added in slot order 0..8.
+// The partial sum of one image slot runs k -> j -> i over the 3x3x3 taps that carry the
+// component's non-zero weights (skipping a zero term is exact).  A tap outside the clamp has
+// weight 0 and is read at a clamped in-bounds index (adding +-0 is exact).
 // ---------------------------------------------------------------------------------------
+// Weights of one staggering along one direction: STAG = 0 centred (kernel*s, taps na..na+2),
+// STAG = 1 staggered (kernel*, taps 1..3); zero outside the clamped window.  Returns false if
+// the window misses [lo,hi].
+template <int STAG>
+__device__ __forceinline__ bool axis1(double c, int lo, int hi, int& first, double w[3]) {
+  int w0, a, b;
+  if (!window(c, lo, hi, w0, a, b)) return false;
+  int t0 = 1;
+  if (STAG == 0) {
+    const double rs0 = (1.0 * (double)w0) - c;
+    t0 = (rs0 > -1.5) ? 0 : 1;
+  }
+  first = w0 + t0;
+#pragma unroll
+  for (int t = 0; t < 3; ++t) {
+    const int i = first + t;
+    const double rs = (1.0 * (double)i) - c;
+    w[t] = (i >= a && i <= b) ? phi(STAG == 0 ? rs : rs - 0.5) : 0.0;
+  }
+  return true;
+}
+
+// One thread per (point, velocity component); the component is blockIdx.y (0: u, 1: v, 2: w).
+// For every image slot whose clamped window meets the grid, all 27 taps of the component's
+// stencil are loaded first, then summed k -> j -> i in the reference's order; partials are
+// added in slot order 0..8.
+template <int C>
+__device__ __forceinline__ double interp_component(const GridD& g, const double* __restrict__ f,
+                                                   double xq, double yq, double zq) {
+  constexpr int SX = C == 0 ? 0 : 1, SY = C == 1 ? 0 : 1, SZ = C == 2 ? 0 : 1;
+  double wz[3];
+  int kz;
+  if (!axis1<SZ>(zq * g.dzi, 1, g.ktot, kz, wz)) return 0.0;
+  int okx = 0, oky = 0;
+#pragma unroll
+  for (int sh = 0; sh < 3; ++sh) {
+    int w0, a, b;
+    if (window(shifted_coord(xq, sh, g.lx, g.bl, g.dxi), 1, g.itot, w0, a, b)) okx |= 1 << sh;
+    if (window(shifted_coord(yq, sh, g.ly, g.bf, g.dyi), 1, g.jtot, w0, a, b)) oky |= 1 << sh;
+  }
+  double sum = 0.0;
+  for (int s = 0; s < 9; ++s) {
+    const int xs = slot_xs(s), ys = slot_ys(s);
+    if (!((okx >> xs) & 1) || !((oky >> ys) & 1)) continue;
+    double wx[3], wy[3];
+    int ix0, jy0;
+    axis1<SX>(shifted_coord(xq, xs, g.lx, g.bl, g.dxi), 1, g.itot, ix0, wx);
+    axis1<SY>(shifted_coord(yq, ys, g.ly, g.bf, g.dyi), 1, g.jtot, jy0, wy);
+    long long ix[3], iy[3], iz[3];
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+      int i = ix0 + t, j = jy0 + t, k = kz + t;  // clamped in-bounds address for zero taps
+      i = i < 0 ? 0 : (i > g.itot + 1 ? g.itot + 1 : i);
+      j = j < 0 ? 0 : (j > g.jtot + 1 ? g.jtot + 1 : j);
+      k = k < 0 ? 0 : (k > g.ktot + 1 ? g.ktot + 1 : k);
+      ix[t] = i, iy[t] = (long long)j * g.sy, iz[t] = (long long)k * g.sz;
+    }
+    double v[27];
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) v[ii + 3 * jj + 9 * kk] = f[ix[ii] + iy[jj] + iz[kk]];
+    double p = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk)
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj)
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) p = p + v[ii + 3 * jj + 9 * kk] * wx[ii] * wy[jj] * wz[kk];
+    sum = sum + p;
+  }
+  return sum;
+}
+
 __global__ void __launch_bounds__(256) k_interp(GridD g, const double* __restrict__ u,
                                                 const double* __restrict__ v,
                                                 const double* __restrict__ w, long long npts,
@@ -115,94 +205,70 @@ __global__ void __launch_bounds__(256) k_interp(GridD g, const double* __restric
                                                 const double* __restrict__ z,
                                                 double* __restrict__ ul, double* __restrict__ vl,
                                                 double* __restrict__ wl, int* __restrict__ err) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= npts) return;
-  double xq = x[q], yq = y[q], zq = z[q];
-  double su = 0.0, sv = 0.0, sw = 0.0;
+  const int c = blockIdx.y;
+  const double xq = x[q], yq = y[q], zq = z[q];
   if (!(isfinite(xq) && isfinite(yq) && isfinite(zq))) {
-    atomicOr(err, ERR_NONFINITE);
-    ul[q] = 0.0, vl[q] = 0.0, wl[q] = 0.0;
+    if (c == 0) atomicOr(err, ERR_NONFINITE);
+    (c == 0 ? ul : (c == 1 ? vl : wl))[q] = 0.0;
     return;
   }
-  double cz = zq * g.dzi;
-  int zw0, za, zb;
-  bool zin = window(cz, 1, g.ktot, zw0, za, zb);
-  if (zin) {
-    double zns[4], zst[4];
-    weights4(cz, zw0, za, zb, zns, zst);
-    for (int s = 0; s < 9; ++s) {
-      double cx, cy;
-      slot_coords(g, s, xq, yq, cx, cy);
-      int xw0, xa, xb, yw0, ya, yb;
-      if (!window(cx, 1, g.itot, xw0, xa, xb)) continue;
-      if (!window(cy, 1, g.jtot, yw0, ya, yb)) continue;
-      double xns[4], xst[4], yns[4], yst[4];
-      weights4(cx, xw0, xa, xb, xns, xst);
-      weights4(cy, yw0, ya, yb, yns, yst);
-      const long long base = (long long)xw0 + (long long)yw0 * g.sy + (long long)zw0 * g.sz;
-      double pu = 0.0, pv = 0.0, pw = 0.0;
-      // u: kernelxs * kernely * kernelz   (interp_spread.f90:370)
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        if (zst[kk] == 0.0) continue;
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-          if (yst[jj] == 0.0) continue;
-#pragma unroll
-          for (int ii = 0; ii < 4; ++ii) {
-            if (xns[ii] == 0.0) continue;
-            pu = pu + u[base + ii + jj * g.sy + kk * g.sz] * xns[ii] * yst[jj] * zst[kk];
-          }
-        }
-      }
-      // v: kernelx * kernelys * kernelz   (:371)
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        if (zst[kk] == 0.0) continue;
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-          if (yns[jj] == 0.0) continue;
-#pragma unroll
-          for (int ii = 0; ii < 4; ++ii) {
-            if (xst[ii] == 0.0) continue;
-            pv = pv + v[base + ii + jj * g.sy + kk * g.sz] * xst[ii] * yns[jj] * zst[kk];
-          }
-        }
-      }
-      // w: kernelx * kernely * kernelzs   (:372)
-#pragma unroll
-      for (int kk = 0; kk < 4; ++kk) {
-        if (zns[kk] == 0.0) continue;
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) {
-          if (yst[jj] == 0.0) continue;
-#pragma unroll
-          for (int ii = 0; ii < 4; ++ii) {
-            if (xst[ii] == 0.0) continue;
-            pw = pw + w[base + ii + jj * g.sy + kk * g.sz] * xst[ii] * yst[jj] * zns[kk];
-          }
-        }
-      }
-      su = su + pu;
-      sv = sv + pv;
-      sw = sw + pw;
-    }
-  }
-  ul[q] = su;
-  vl[q] = sv;
-  wl[q] = sw;
+  // u: kernelxs*kernely*kernelz, v: kernelx*kernelys*kernelz, w: kernelx*kernely*kernelzs
+  // (interp_spread.f90:370-372)
+  if (c == 0)
+    ul[q] = interp_component<0>(g, u, xq, yq, zq);
+  else if (c == 1)
+    vl[q] = interp_component<1>(g, v, xq, yq, zq);
+  else
+    wl[q] = interp_component<2>(g, w, xq, yq, zq);
+}
+
+// ---------------------------------------------------------------------------------------
+// Spread tiles are cut along the aligned 128-byte chunks of the field's memory, so that
+// every tile writes whole chunks.  In row (j,k) the memory index of cell i is base + i with
+// base = j*(itot+2) + k*(itot+2)(jtot+2); the row phase is s = base mod 16 and cell i lies in
+// tile column (i + s) / 16 at position (i + s) mod 16.  A tile is one column over TJ x TK
+// rows: 16 positions per row, whose cells shift with the row phase.  The periodic halo
+// cells i = 0 and i = itot+1 of a row are tiled like the others and computed as duplicates
+// of cells itot and 1 (their values after bound.f90:141-190).
+// ---------------------------------------------------------------------------------------
+constexpr int SP_CW = 16;  // doubles per aligned chunk = tile column width
+
+__device__ __forceinline__ int row_phase(const GridD& g, int j, int k) {
+  return (int)(((long long)j * g.sy + (long long)k * g.sz) & (SP_CW - 1));
 }
 
 // ---------------------------------------------------------------------------------------
 // Spread, step 1: binning.  Every (point, image slot) pair whose clamped window meets the
-// grid is one candidate; it is listed in every tile its window overlaps (at most 2 per
-// direction).  Its 64-bit entry is (K << 32) | (point | nb << 28) where
+// grid is one candidate; it is listed in every tile whose cells its window covers: for each
+// tile row-block (tj, tk) it meets, the columns that hold its x-taps in those rows, plus the
+// columns of the halo duplicates (cell i = 0 when the window covers itot, cell itot+1 when it
+// covers 1).  Its 64-bit entry is (K << 32) | (point | nb << 28) where
 // K = 9 off_p + nb n_p + l orders candidates exactly as the reference's loop nest visits
 // them: capsule p, then image loop index nb, then point l (interp_spread.f90:825-878).
-// MODE 0 counts per tile; MODE 1 writes entries at tile_off[t] + cursor[t].
+// MODE 0 counts per tile; MODE 1 writes entries at tile_off[t] + cursor[t].  Lanes of a
+// wavefront that target the same tile share one atomic (wave-aggregated).
 // ---------------------------------------------------------------------------------------
+__device__ __forceinline__ unsigned agg_add(unsigned* __restrict__ cnt, int tile, bool valid) {
+  unsigned long long active = __ballot(valid);
+  unsigned res = 0;
+  while (active) {
+    const int leader = __ffsll((long long)active) - 1;
+    const int lt = __shfl(tile, leader);
+    const bool mine = valid && tile == lt;
+    const unsigned long long peers = __ballot(mine);
+    unsigned base = 0;
+    if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(&cnt[lt], (unsigned)__popcll(peers));
+    base = __shfl(base, leader);
+    if (mine) res = base + (unsigned)__popcll(peers & __lanemask_lt());
+    active &= ~peers;
+  }
+  return res;
+}
+
 template <int MODE>
-__global__ void __launch_bounds__(256) k_bin(GridD g, TileGeom tg, bool cc, long long ncap,
+__global__ void __launch_bounds__(256) k_bin(GridD g, TileGeom tg, long long ncap,
                                              const long long* __restrict__ cap_off,
                                              long long npts, const double* __restrict__ x,
                                              const double* __restrict__ y,
@@ -212,83 +278,227 @@ __global__ void __launch_bounds__(256) k_bin(GridD g, TileGeom tg, bool cc, long
                                              unsigned long long* __restrict__ entries,
                                              unsigned long long ecap,
                                              int* __restrict__ err) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= npts) return;
-  double xq = x[q], yq = y[q], zq = z[q];
-  if (!(isfinite(xq) && isfinite(yq) && isfinite(zq))) {
-    if (MODE == 0) atomicOr(err, ERR_NONFINITE);
-    return;
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  bool live = q < npts;
+  double xq = 0, yq = 0, zq = 0;
+  if (live) {
+    xq = x[q], yq = y[q], zq = z[q];
+    if (!(isfinite(xq) && isfinite(yq) && isfinite(zq))) {
+      if (MODE == 0) atomicOr(err, ERR_NONFINITE);
+      live = false;
+    }
   }
-  double cz = zq * g.dzi;
-  int zw0, za, zb;
-  if (!window(cz, tg.klo, tg.khi, zw0, za, zb)) return;
-  // capsule of point q: largest p with cap_off[p] <= q
-  long long lo = 0, hi = ncap - 1;
-  while (lo < hi) {
-    long long mid = (lo + hi + 1) >> 1;
-    if (cap_off[mid] <= q)
-      lo = mid;
-    else
-      hi = mid - 1;
+  int zw0 = 0, za = 1, zb = 0;
+  if (live) live = window(zq * g.dzi, tg.klo, tg.khi, zw0, za, zb);
+  // clamped windows of the x / y copies (unshifted, +L, -L); a > b marks "misses the grid"
+  int xa[3], xb[3], ya[3], yb[3];
+#pragma unroll
+  for (int sh = 0; sh < 3; ++sh) {
+    int w0;
+    xa[sh] = ya[sh] = 1, xb[sh] = yb[sh] = 0;
+    if (live && !window(shifted_coord(xq, sh, g.lx, g.bl, g.dxi), 1, g.itot, w0, xa[sh], xb[sh]))
+      xa[sh] = 1, xb[sh] = 0;
+    if (live && !window(shifted_coord(yq, sh, g.ly, g.bf, g.dyi), 1, g.jtot, w0, ya[sh], yb[sh]))
+      ya[sh] = 1, yb[sh] = 0;
   }
-  const long long p = lo;
-  const long long offp = cap_off[p];
-  const long long np_ = cap_off[p + 1] - offp;
-  const long long l = q - offp;
-  const int tk0 = (za - tg.kbase) / tg.tk, tk1 = (zb - tg.kbase) / tg.tk;
+  const int tk0 = live ? (za - tg.kbase) / tg.tk : 0;
+  unsigned long long offp = 0, np_ = 0, l = 0;
+  if (MODE == 1 && live) {
+    long long lo = 0, hi = ncap - 1;  // capsule of point q: largest p with cap_off[p] <= q
+    while (lo < hi) {
+      long long mid = (lo + hi + 1) >> 1;
+      if (cap_off[mid] <= q)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    offp = (unsigned long long)cap_off[lo];
+    np_ = (unsigned long long)(cap_off[lo + 1] - cap_off[lo]);
+    l = (unsigned long long)q - offp;
+  }
   for (int nb = 0; nb < 9; ++nb) {
-    int s = nb_slot(nb);
-    double cx, cy;
-    slot_coords(g, s, xq, yq, cx, cy);
-    int xw0, xa, xb, yw0, ya, yb;
-    if (!window(cx, 1, g.itot, xw0, xa, xb)) continue;
-    if (!window(cy, 1, g.jtot, yw0, ya, yb)) continue;
-    const int ti0 = (xa - 1) / tg.ti, ti1 = (xb - 1) / tg.ti;
-    const int tj0 = (ya - 1) / tg.tj, tj1 = (yb - 1) / tg.tj;
+    const int s = nb_slot(nb);
+    const int xs = slot_xs(s), ys = slot_ys(s);
+    const bool has = live && xa[xs] <= xb[xs] && ya[ys] <= yb[ys];
+    if (!__ballot(has)) continue;
     unsigned long long key = 0;
     if (MODE == 1) {
-      unsigned long long K = 9ull * (unsigned long long)offp +
-                             (unsigned long long)nb * (unsigned long long)np_ +
-                             (unsigned long long)l;
+      const unsigned long long K = 9ull * offp + (unsigned long long)nb * np_ + l;
       key = (K << 32) | (unsigned long long)((unsigned)q | ((unsigned)nb << 28));
     }
-    for (int tk = tk0; tk <= tk1; ++tk)
-      for (int tj = tj0; tj <= tj1; ++tj)
-        for (int ti = ti0; ti <= ti1; ++ti) {
-          long long t = (long long)ti + (long long)tg.nti * ((long long)tj + (long long)tg.ntj * tk);
-          if (MODE == 0) {
-            atomicAdd(&count[t], 1u);
-          } else {
-            unsigned pos = tile_off[t] + atomicAdd(&count[t], 1u);
-            if ((unsigned long long)pos < ecap)
-              entries[pos] = key;
-            else
-              atomicOr(err, ERR_CAPACITY);
+    const int cxa = has ? xa[xs] : 1, cxb = has ? xb[xs] : 0;
+    const int cya = has ? ya[ys] : 1, cyb = has ? yb[ys] : 0;
+    const int tj0 = (cya - 1) / tg.tj;
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+      const int tj = tj0 + (rb & 1), tk = tk0 + (rb >> 1);
+      // rows of this row-block inside the window
+      const int j0 = max(cya, 1 + tj * tg.tj), j1 = min(cyb, (tj + 1) * tg.tj);
+      const int k0 = max(za, tg.kbase + tk * tg.tk), k1 = min(zb, tg.kbase + (tk + 1) * tg.tk - 1);
+      const bool rv = has && j0 <= j1 && k0 <= k1;
+      int smin = SP_CW, smax = -1;
+      if (rv)
+        for (int k = k0; k <= k1; ++k)
+          for (int j = j0; j <= j1; ++j) {
+            const int ph = row_phase(g, j, k);
+            smin = min(smin, ph), smax = max(smax, ph);
           }
+      // candidate columns: the window's taps (<= 3 columns), the duplicate of cell itot
+      // (column 0) and the duplicate of cell 1 (<= 2 columns at the row end)
+      int col[6];
+      bool cv[6];
+      const int cm0 = (cxa + smin) / SP_CW, cm1 = (cxb + smax) / SP_CW;
+      const int ch0 = (g.itot + 1 + smin) / SP_CW, ch1 = (g.itot + 1 + smax) / SP_CW;
+#pragma unroll
+      for (int u = 0; u < 3; ++u) col[u] = cm0 + u, cv[u] = rv && cm0 + u <= cm1;
+      col[3] = 0, cv[3] = rv && cxb == g.itot;
+      col[4] = ch0, cv[4] = rv && cxa == 1;
+      col[5] = ch0 + 1, cv[5] = rv && cxa == 1 && ch0 + 1 <= ch1;
+#pragma unroll
+      for (int u = 3; u < 6; ++u)
+#pragma unroll
+        for (int w = 0; w < u; ++w)
+          if (cv[w] && col[w] == col[u]) cv[u] = false;
+#pragma unroll
+      for (int u = 0; u < 6; ++u) {
+        const bool val = cv[u];
+        const int t = val ? col[u] + tg.nti * (tj + tg.ntj * tk) : 0;
+        const unsigned pos = agg_add(count, t, val);
+        if (MODE == 1 && val) {
+          const unsigned long long at = (unsigned long long)tile_off[t] + pos;
+          if (at < ecap)
+            entries[at] = key;
+          else
+            atomicOr(err, ERR_CAPACITY);
         }
+      }
+    }
   }
 }
 
 // ---------------------------------------------------------------------------------------
-// Spread, step 2: one workgroup per tile.
-//   (a) load the tile's candidate entries and sort them by K (bitonic, in LDS);
-//   (b) in chunks of 64 candidates: compute each candidate's window and its 24 weights;
-//       mark, for every cell of the tile, which chunk candidates touch it (one 64-bit mask
-//       per cell, integer LDS atomics -- the set is order-independent);
-//   (c) every thread walks the set bits of its cells in ascending order, i.e. in K order,
-//       and adds the terms to register accumulators: cell (i,j,k) receives exactly the
-//       reference's sequence force = force + f*wx*wy*wz*(area/dVeul) (:869-874 / :1334-1338);
-//   (d) the tile is written once, with the periodic halo copies of bound.f90:141-190.
-// No floating-point atomics: results are deterministic and bit-identical to the reference.
+// Exclusive scan of the per-tile counts, fused with the compaction of the non-empty tiles.
+// Each tile contributes the packed value count | (count > 0) << 32, so one 64-bit scan gives
+// both the list offset (low word) and the tile's rank among non-empty tiles (high word).
+// Integer arithmetic: order-independent.
 // ---------------------------------------------------------------------------------------
-constexpr int SP_THREADS = 256;
-constexpr int SP_CH = 64;      // candidates per chunk (one bit each in a 64-bit mask)
-constexpr int SP_CAPK = 2048;  // candidates sorted in LDS; larger lists are sorted in place in HBM
+constexpr int SCAN_B = 1024;  // elements per block (256 threads x 4)
 
-// Ascending sort of a[0..n) (n <= P, P a power of two) with the "flip" form of the bitonic
-// network: every compare-exchange puts the minimum at the lower index, so the virtual +inf
-// elements a[n..P) never move and need no storage.  Works in LDS or in HBM (one workgroup;
-// __syncthreads orders the workgroup's own global accesses).
+__device__ __forceinline__ unsigned long long block_excl_scan(unsigned long long v,
+                                                              unsigned long long* sh,
+                                                              unsigned long long& total) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    unsigned long long t = tid >= o ? sh[tid - o] : 0ull;
+    __syncthreads();
+    sh[tid] += t;
+    __syncthreads();
+  }
+  total = sh[255];
+  unsigned long long incl = sh[tid];
+  __syncthreads();
+  return incl - v;
+}
+
+__device__ __forceinline__ unsigned long long packc(unsigned c) {
+  return (unsigned long long)c | (c ? (1ull << 32) : 0ull);
+}
+
+__global__ void __launch_bounds__(256) k_scan_reduce(const unsigned* __restrict__ in, long long n,
+                                                     unsigned long long* __restrict__ part) {
+  __shared__ unsigned long long sh[256];
+  long long b0 = (long long)blockIdx.x * SCAN_B + threadIdx.x * 4;
+  unsigned long long s = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    if (b0 + r < n) s += packc(in[b0 + r]);
+  unsigned long long tot;
+  block_excl_scan(s, sh, tot);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256) k_scan_parts(unsigned long long* __restrict__ part,
+                                                    long long nb) {
+  __shared__ unsigned long long sh[256];
+  unsigned long long carry = 0;
+  for (long long c = 0; c < nb; c += 1024) {
+    long long b0 = c + threadIdx.x * 4;
+    unsigned long long v[4], s = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      v[r] = (b0 + r < nb) ? part[b0 + r] : 0ull;
+      s += v[r];
+    }
+    unsigned long long tot;
+    unsigned long long ex = block_excl_scan(s, sh, tot) + carry;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (b0 + r < nb) part[b0 + r] = ex;
+      ex += v[r];
+    }
+    carry += tot;
+    __syncthreads();
+  }
+}
+
+// off[0..n]: candidate list offsets; list[0..nne): non-empty tiles in increasing order;
+// meta[0] = nne, meta[1..287] = 0 (work-queue counters of the spread kernel, reset here).
+__global__ void __launch_bounds__(256) k_scan_apply(const unsigned* __restrict__ in, long long n,
+                                                    const unsigned long long* __restrict__ part,
+                                                    unsigned* __restrict__ off,
+                                                    int* __restrict__ list,
+                                                    int* __restrict__ meta) {
+  __shared__ unsigned long long sh[256];
+  long long b0 = (long long)blockIdx.x * SCAN_B + threadIdx.x * 4;
+  unsigned c[4];
+  unsigned long long s = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    c[r] = (b0 + r < n) ? in[b0 + r] : 0u;
+    s += packc(c[r]);
+  }
+  unsigned long long tot;
+  unsigned long long ex = block_excl_scan(s, sh, tot) + part[blockIdx.x];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (b0 + r < n) {
+      off[b0 + r] = (unsigned)(ex & 0xffffffffull);
+      if (c[r]) list[ex >> 32] = (int)(b0 + r);
+    }
+    ex += packc(c[r]);
+  }
+  if (b0 <= n - 1 && n - 1 < b0 + 4) {
+    const unsigned long long all = part[blockIdx.x] + tot;
+    off[n] = (unsigned)(all & 0xffffffffull);
+    meta[0] = (int)(all >> 32);
+    for (int q = 1; q < 9 * 32; ++q) meta[q] = 0;  // work-queue counters of the spread kernel
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Spread, step 2: one workgroup per tile, for every tile of the grid, taken in tile order
+// from one work queue per XCD over a contiguous range of tiles.  A tile without candidates
+// is written as zeros.  A tile with candidates:
+//   (a) loads its candidate entries and sorts them by K (rank sort in LDS; bitonic in place
+//       in HBM beyond SP_CAPK);
+//   (b) in chunks of SP_CH candidates: computes each candidate's window and weights, and
+//       marks for every cell of the tile which chunk candidates touch it (one bit each,
+//       integer LDS atomics -- the set is order-independent);
+//   (c) every thread walks the set bits of its cells in ascending order, i.e. in K order, and
+//       adds the terms to register accumulators: cell (i,j,k) receives exactly the
+//       reference's sequence force = force + f*wx*wy*wz*(area/dVeul) (:869-874 / :1334-1338);
+//   (d) writes the tile once: whole aligned 128-byte chunks, 16-byte stores.
+// No floating-point atomics: results are deterministic and bit-identical to the reference.
+// The j halo rows are copied afterwards by k_jhalo (bound.f90:172-177).
+// ---------------------------------------------------------------------------------------
+constexpr int SP_THREADS = 256; // threads per spread workgroup
+constexpr int SP_CH = 64;       // candidates per chunk (one bit each in a 64-bit mask)
+constexpr int SP_CAPK = 512;    // candidates sorted in LDS; larger lists are sorted in HBM
+constexpr int SP_TG = 8;        // tiles per work item
+constexpr int SP_QSTRIDE = 32;  // ints between the per-XCD queue counters (128 bytes)
+
 __device__ __forceinline__ void cmpx(unsigned long long* a, int lo, int hi, int n) {
   if (hi < n) {
     unsigned long long x = a[lo], y = a[hi];
@@ -299,6 +509,10 @@ __device__ __forceinline__ void cmpx(unsigned long long* a, int lo, int hi, int 
   }
 }
 
+// Ascending sort of a[0..n) with the "flip" form of the bitonic network: every
+// compare-exchange puts the minimum at the lower index, so the virtual +inf elements
+// a[n..P) never move and need no storage.  One workgroup; __syncthreads orders the
+// workgroup's own accesses (LDS or HBM).
 __device__ void bitonic_sort(unsigned long long* a, int n, int tid) {
   int P = 1;
   while (P < n) P <<= 1;
@@ -320,24 +534,47 @@ __device__ void bitonic_sort(unsigned long long* a, int n, int tid) {
   }
 }
 
-template <int TI, int TJ, int TK, bool CC>
+template <int TJ, int TK, bool CC>
 struct SpreadTile {
-  static constexpr int NC = TI * TJ * TK;
-  static constexpr int CPT = NC / SP_THREADS;
-  static_assert(NC % SP_THREADS == 0, "tile must hold a multiple of 256 cells");
-  static_assert(TI >= 4 && TJ >= 4 && TK >= 4, "tile must be at least the stencil width");
+  static constexpr int NR = TJ * TK;         // rows per tile
+  static constexpr int NC = SP_CW * NR;      // cells per tile
+  static constexpr int NP = NC / 2;          // cell pairs per tile
+  static constexpr int PPT = NP / SP_THREADS;  // pairs per thread
+  static_assert(NP % SP_THREADS == 0, "tile must hold a multiple of 512 cells");
+  static_assert(TJ >= 4 && TK >= 4 && TJ <= 64 && TK <= 64, "row block 4..64 each way");
 
   struct Smem {
     unsigned long long keys[SP_CAPK];
     unsigned long long mask[NC];
-    double wt[SP_CH][3][8];  // per candidate, per direction: ns[4], st[4]
-    double fv[SP_CH][4];     // f1, f2, f3, area/dVeul
-    int w0[SP_CH][4];        // window origin relative to the tile, per direction; [3]: valid
-    int lim[SP_CH][6];       // clamped window [a,b] relative to the tile, per direction
+    double2 wp[SP_CH][3][4];  // per candidate, direction, tap: (centred, staggered) weight
+    double2 fa[SP_CH][2];     // (f1, f2), (f3, area/dVeul)
+    int wx0[SP_CH];           // x window origin (virtual i of its first tap)
+    int xr[SP_CH];            // clamped x window: a | b << 16
+    union {
+      int w;
+      unsigned char b[4];
+    } org[SP_CH];             // y, z window origin relative to the tile (bytes 1, 2), +8
+    int box[SP_CH][2];        // clamped y, z window within the tile: a | b << 8
+    int item;
   };
 
-  __device__ static void run(Smem& sm, long long t, const GridD& g, const TileGeom& tg,
-                             const unsigned* __restrict__ tile_off,
+  // cell c of the tile (position p = c % 16, row r = c / 16): grid coordinates; returns
+  // false for positions outside the row or rows outside the grid
+  __device__ static bool cell_of(const GridD& g, const TileGeom& tg, int col, int j0, int k0,
+                                 int c, int& i, int& vi, int& lj, int& lk, long long& m) {
+    const int p = c % SP_CW, r = c / SP_CW;
+    lj = r % TJ, lk = r / TJ;
+    const int j = j0 + lj, k = k0 + lk;
+    const long long base = (long long)j * g.sy + (long long)k * g.sz;
+    const int s = (int)(base & (SP_CW - 1));
+    i = SP_CW * col + p - s;
+    m = base + i;
+    vi = i == 0 ? g.itot : (i == g.itot + 1 ? 1 : i);
+    return j <= g.jtot && k < tg.kbase + tg.nkp && i >= 0 && i <= g.itot + 1;
+  }
+
+  __device__ static void run(Smem& sm, long long t, int parts, const GridD& g,
+                             const TileGeom& tg, const unsigned* __restrict__ tile_off,
                              unsigned long long* __restrict__ entries,
                              unsigned long long ecap, const double* __restrict__ x,
                              const double* __restrict__ y, const double* __restrict__ z,
@@ -346,23 +583,38 @@ struct SpreadTile {
                              double* __restrict__ o1, double* __restrict__ o2,
                              double* __restrict__ o3) {
     const int tid = threadIdx.x;
-    const int ti = (int)(t % tg.nti);
+    const int col = (int)(t % tg.nti);
     const int tj = (int)((t / tg.nti) % tg.ntj);
     const int tk = (int)(t / ((long long)tg.nti * tg.ntj));
-    const int i0 = 1 + ti * TI, j0 = 1 + tj * TJ, k0 = tg.kbase + tk * TK;
+    const int j0 = 1 + tj * TJ, k0 = tg.kbase + tk * TK;
 
-    unsigned base = tile_off[t];
+    const unsigned base = tile_off[t];
     unsigned long long endl = tile_off[t + 1];
     if (endl > ecap) endl = ecap;  // overflowed lists are flagged by k_bin; never read past
-    int n = (unsigned long long)base < endl ? (int)(endl - base) : 0;
+    const int n = (unsigned long long)base < endl ? (int)(endl - base) : 0;
+    if (!(parts & (n > 0 ? 1 : 2))) return;
 
-    double acc1[CPT], acc2[CPT], acc3[CPT];
+    // this thread's cells: pairs tid + h*256, cells 2*pair and 2*pair+1
+    double acc1[2 * PPT], acc2[2 * PPT], acc3[2 * PPT];
 #pragma unroll
-    for (int r = 0; r < CPT; ++r) acc1[r] = acc2[r] = acc3[r] = 0.0;
+    for (int r = 0; r < 2 * PPT; ++r) acc1[r] = acc2[r] = acc3[r] = 0.0;
 
     if (n > 0) {
       const unsigned long long* src;
-      if (n <= SP_CAPK) {
+      if (n <= SP_THREADS) {
+        // rank sort: keys are unique, rank = number of smaller keys
+        unsigned long long* in = sm.keys + SP_THREADS;
+        const unsigned long long mine = tid < n ? entries[base + tid] : 0ull;
+        in[tid] = mine;
+        __syncthreads();
+        if (tid < n) {
+          int rank = 0;
+          for (int j = 0; j < n; ++j) rank += in[j] < mine ? 1 : 0;
+          sm.keys[rank] = mine;
+        }
+        __syncthreads();
+        src = sm.keys;
+      } else if (n <= SP_CAPK) {
         for (int i = tid; i < n; i += SP_THREADS) sm.keys[i] = entries[base + i];
         __syncthreads();
         bitonic_sort(sm.keys, n, tid);
@@ -374,178 +626,213 @@ struct SpreadTile {
       for (int c0 = 0; c0 < n; c0 += SP_CH) {
         const int m = (n - c0) < SP_CH ? (n - c0) : SP_CH;
 #pragma unroll
-        for (int r = 0; r < CPT; ++r) sm.mask[tid + r * SP_THREADS] = 0ull;
-        // (b) candidate preparation: 4 threads per candidate
-        {
-          const int c = tid >> 2, d = tid & 3;
-          if (c < m) {
-            unsigned long long e = src[c0 + c];
-            unsigned lo32 = (unsigned)(e & 0xffffffffull);
-            long long q = (long long)(lo32 & 0x0fffffffu);
-            int nb = (int)(lo32 >> 28);
-            int s = nb_slot(nb);
-            if (d < 3) {
-              double cc_;
-              int lo, hi, base_;
-              if (d == 0) {
-                double cy_;
-                slot_coords(g, s, x[q], y[q], cc_, cy_);
-                lo = 1, hi = g.itot, base_ = i0;
-              } else if (d == 1) {
-                double cx_;
-                slot_coords(g, s, x[q], y[q], cx_, cc_);
-                lo = 1, hi = g.jtot, base_ = j0;
-              } else {
-                cc_ = z[q] * g.dzi;
-                lo = tg.klo, hi = tg.khi, base_ = k0;
-              }
-              int w0, a, b;
-              bool ok = window(cc_, lo, hi, w0, a, b);
-              double ns[4], st[4];
-              if (ok) {
-                weights4(cc_, w0, a, b, ns, st);
-              } else {
-#pragma unroll
-                for (int tt = 0; tt < 4; ++tt) ns[tt] = st[tt] = 0.0;
-                w0 = 0, a = 1, b = 0;
-              }
-#pragma unroll
-              for (int tt = 0; tt < 4; ++tt) {
-                sm.wt[c][d][tt] = ns[tt];
-                sm.wt[c][d][4 + tt] = st[tt];
-              }
-              sm.w0[c][d] = w0 - base_;
-              sm.lim[c][2 * d] = a - base_;
-              sm.lim[c][2 * d + 1] = b - base_;
+        for (int r = 0; r < NC / SP_THREADS; ++r) sm.mask[tid + r * SP_THREADS] = 0ull;
+        // (b) candidate preparation: 4 threads per candidate (x, y, z weights; values)
+        for (int c = tid >> 2; c < m; c += SP_THREADS / 4) {
+          const int d = tid & 3;
+          const unsigned long long e = src[c0 + c];
+          const unsigned lo32 = (unsigned)(e & 0xffffffffull);
+          const long long q = (long long)(lo32 & 0x0fffffffu);
+          const int s = nb_slot((int)(lo32 >> 28));
+          if (d < 3) {
+            double cc_;
+            int lo, hi, base_, ext;
+            if (d == 0) {
+              cc_ = shifted_coord(x[q], slot_xs(s), g.lx, g.bl, g.dxi);
+              lo = 1, hi = g.itot, base_ = 0, ext = 1 << 30;
+            } else if (d == 1) {
+              cc_ = shifted_coord(y[q], slot_ys(s), g.ly, g.bf, g.dyi);
+              lo = 1, hi = g.jtot, base_ = j0, ext = TJ;
             } else {
-              sm.fv[c][0] = f1[q];
-              sm.fv[c][1] = f2[q];
-              sm.fv[c][2] = f3[q];
-              sm.fv[c][3] = area[q] / g.dveul;
+              cc_ = z[q] * g.dzi;
+              lo = tg.klo, hi = tg.khi, base_ = k0, ext = TK;
             }
+            Axis ax;
+            int w0 = 0, bl = 1, bh = 0;
+            if (axis_of(cc_, lo, hi, ax)) {
+              const bool n0 = ax.na == 0;
+              sm.wp[c][d][0] = make_double2(n0 ? ax.ns[0] : 0.0, 0.0);
+              sm.wp[c][d][1] = make_double2(n0 ? ax.ns[1] : ax.ns[0], ax.st[0]);
+              sm.wp[c][d][2] = make_double2(n0 ? ax.ns[2] : ax.ns[1], ax.st[1]);
+              sm.wp[c][d][3] = make_double2(n0 ? 0.0 : ax.ns[2], ax.st[2]);
+              int a, b;
+              window(cc_, lo, hi, w0, a, b);
+              w0 -= base_;
+              bl = max(a - base_, 0);
+              bh = min(b - base_, ext - 1);
+            }
+            if (d == 0) {
+              sm.wx0[c] = w0;
+              sm.xr[c] = bl | (bh << 16);
+            } else {
+              sm.box[c][d - 1] = bl | (bh << 8);
+              sm.org[c].b[d] = (unsigned char)(w0 + 8);
+            }
+          } else {
+            sm.fa[c][0] = make_double2(f1[q], f2[q]);
+            sm.fa[c][1] = make_double2(f3[q], area[q] / g.dveul);
           }
         }
         __syncthreads();
-        // mark the cells each candidate touches: candidate c = tid/4 marks 16 of its 64
-        {
-          const int c = tid >> 2, part = tid & 3;
-          if (c < m) {
-            const int wx = sm.w0[c][0], wy = sm.w0[c][1], wz = sm.w0[c][2];
-            const int ax = max(sm.lim[c][0], 0), bx = min(sm.lim[c][1], TI - 1);
-            const int ay = max(sm.lim[c][2], 0), by = min(sm.lim[c][3], TJ - 1);
-            const int az = max(sm.lim[c][4], 0), bz = min(sm.lim[c][5], TK - 1);
+        // mark the cells each candidate touches: 4 threads per candidate, one x-tap each,
+        // over the window's rows within the tile; a tap at i = itot also marks the halo
+        // duplicate at i = 0, a tap at i = 1 the one at i = itot+1
+        for (int c = tid >> 2; c < m; c += SP_THREADS / 4) {
+          const int part = tid & 3;
+          const int xr = sm.xr[c];
+          const int vi = (xr & 0xffff) + part;
+          if (vi <= (xr >> 16)) {
             const unsigned long long bit = 1ull << c;
-#pragma unroll
-            for (int h = 0; h < 16; ++h) {
-              const int idx = part * 16 + h;
-              const int li = wx + (idx & 3), lj = wy + ((idx >> 2) & 3), lk = wz + (idx >> 4);
-              if (li >= ax && li <= bx && lj >= ay && lj <= by && lk >= az && lk <= bz)
-                atomicOr(&sm.mask[li + TI * (lj + TJ * lk)], bit);
-            }
+            const int by = sm.box[c][0], bz = sm.box[c][1];
+            for (int lk = bz & 0xff; lk <= (bz >> 8); ++lk)
+              for (int lj = by & 0xff; lj <= (by >> 8); ++lj) {
+                const int s = row_phase(g, j0 + lj, k0 + lk);
+                const int rowc = SP_CW * (lj + TJ * lk);
+                const int p = vi + s - SP_CW * col;
+                if (p >= 0 && p < SP_CW) atomicOr(&sm.mask[rowc + p], bit);
+                if (vi == g.itot) {
+                  const int ph = s - SP_CW * col;  // cell i = 0
+                  if (ph >= 0 && ph < SP_CW) atomicOr(&sm.mask[rowc + ph], bit);
+                }
+                if (vi == 1) {
+                  const int ph = g.itot + 1 + s - SP_CW * col;  // cell i = itot+1
+                  if (ph >= 0 && ph < SP_CW) atomicOr(&sm.mask[rowc + ph], bit);
+                }
+              }
           }
         }
         __syncthreads();
-        // (c) ordered accumulation
+        // (c) ordered accumulation: set bits in ascending order = K order
 #pragma unroll
-        for (int r = 0; r < CPT; ++r) {
-          const int cell = tid + r * SP_THREADS;
-          const int li = cell % TI, lj = (cell / TI) % TJ, lk = cell / (TI * TJ);
+        for (int r = 0; r < 2 * PPT; ++r) {
+          const int cell = 2 * (tid + (r >> 1) * SP_THREADS) + (r & 1);
           unsigned long long mk = sm.mask[cell];
+          if (!mk) continue;
+          int i, vi, lj, lk;
+          long long mm;
+          cell_of(g, tg, col, j0, k0, cell, i, vi, lj, lk, mm);
           while (mk) {
             const int c = __builtin_ctzll(mk);
             mk &= mk - 1;
-            const int a = li - sm.w0[c][0], b = lj - sm.w0[c][1], d = lk - sm.w0[c][2];
-            const double adv = sm.fv[c][3];
+            const int o = sm.org[c].w;
+            const int a = vi - sm.wx0[c], b = lj - (((o >> 8) & 0xff) - 8),
+                      d = lk - (((o >> 16) & 0xff) - 8);
+            const double2 wxp = sm.wp[c][0][a], wyp = sm.wp[c][1][b], wzp = sm.wp[c][2][d];
+            const double2 fa0 = sm.fa[c][0], fa1 = sm.fa[c][1];
+            const double adv = fa1.y;
             if (!CC) {
-              const double wxs = sm.wt[c][0][a], wx = sm.wt[c][0][4 + a];
-              const double wys = sm.wt[c][1][b], wy = sm.wt[c][1][4 + b];
-              const double wzs = sm.wt[c][2][d], wz = sm.wt[c][2][4 + d];
-              acc1[r] = acc1[r] + sm.fv[c][0] * wxs * wy * wz * adv;
-              acc2[r] = acc2[r] + sm.fv[c][1] * wx * wys * wz * adv;
-              acc3[r] = acc3[r] + sm.fv[c][2] * wx * wy * wzs * adv;
+              acc1[r] = acc1[r] + fa0.x * wxp.x * wyp.y * wzp.y * adv;
+              acc2[r] = acc2[r] + fa0.y * wxp.y * wyp.x * wzp.y * adv;
+              acc3[r] = acc3[r] + fa1.x * wxp.y * wyp.y * wzp.x * adv;
             } else {
-              const double wx = sm.wt[c][0][4 + a], wy = sm.wt[c][1][4 + b],
-                           wz = sm.wt[c][2][4 + d];
-              acc1[r] = acc1[r] + sm.fv[c][0] * wx * wy * wz * adv;
-              acc2[r] = acc2[r] + sm.fv[c][1] * wx * wy * wz * adv;
-              acc3[r] = acc3[r] + sm.fv[c][2] * wx * wy * wz * adv;
+              acc1[r] = acc1[r] + fa0.x * wxp.y * wyp.y * wzp.y * adv;
+              acc2[r] = acc2[r] + fa0.y * wxp.y * wyp.y * wzp.y * adv;
+              acc3[r] = acc3[r] + fa1.x * wxp.y * wyp.y * wzp.y * adv;
             }
           }
         }
         __syncthreads();
       }
     }
-    // (d) write the tile and its periodic halo images
+    // (d) write: each thread's pairs are two adjacent positions of one row, 16 bytes
 #pragma unroll
-    for (int r = 0; r < CPT; ++r) {
-      const int cell = tid + r * SP_THREADS;
-      const int li = cell % TI, lj = (cell / TI) % TJ, lk = cell / (TI * TJ);
-      const int i = i0 + li, j = j0 + lj, k = k0 + lk;
-      if (i > g.itot || j > g.jtot || k >= tg.kbase + tg.nkp) continue;
-      const long long kk = (long long)k * g.sz;
-      const double a1 = acc1[r], a2 = acc2[r], a3 = acc3[r];
-      long long idx = (long long)i + (long long)j * g.sy + kk;
-      o1[idx] = a1, o2[idx] = a2, o3[idx] = a3;
-      const int ih = (i == 1) ? g.itot + 1 : ((i == g.itot) ? 0 : -1);
-      const int jh = (j == 1) ? g.jtot + 1 : ((j == g.jtot) ? 0 : -1);
-      if (ih >= 0) {
-        long long h = (long long)ih + (long long)j * g.sy + kk;
-        o1[h] = a1, o2[h] = a2, o3[h] = a3;
+    for (int h = 0; h < PPT; ++h) {
+      const int cell = 2 * (tid + h * SP_THREADS);
+      int i0, vi0, lj, lk, i1, vi1, lj1, lk1;
+      long long m0, m1;
+      const bool v0 = cell_of(g, tg, col, j0, k0, cell, i0, vi0, lj, lk, m0);
+      const bool v1 = cell_of(g, tg, col, j0, k0, cell + 1, i1, vi1, lj1, lk1, m1);
+      const int r0 = 2 * h, r1 = 2 * h + 1;
+      if (v0 && v1 && (m0 & 1) == 0) {
+        *reinterpret_cast<double2*>(o1 + m0) = make_double2(acc1[r0], acc1[r1]);
+        *reinterpret_cast<double2*>(o2 + m0) = make_double2(acc2[r0], acc2[r1]);
+        *reinterpret_cast<double2*>(o3 + m0) = make_double2(acc3[r0], acc3[r1]);
+      } else {
+        if (v0) o1[m0] = acc1[r0], o2[m0] = acc2[r0], o3[m0] = acc3[r0];
+        if (v1) o1[m1] = acc1[r1], o2[m1] = acc2[r1], o3[m1] = acc3[r1];
       }
-      if (jh >= 0) {
-        long long h = (long long)i + (long long)jh * g.sy + kk;
-        o1[h] = a1, o2[h] = a2, o3[h] = a3;
-        if (ih >= 0) {
-          long long h2 = (long long)ih + (long long)jh * g.sy + kk;
-          o1[h2] = a1, o2[h2] = a2, o3[h2] = a3;
-        }
-      }
-      // itot == 1 or jtot == 1 would need both images; the context rejects such grids.
     }
   }
 };
 
-// Workgroup -> tile: consecutive workgroups are dealt round-robin to the 8 XCDs, so give
-// each XCD a contiguous run of tiles (neighbouring tiles share candidates in its L2).
-__device__ __forceinline__ long long xcd_tile(long long bid, long long nblk) {
-  long long per = nblk >> 3;
-  return (bid & 7) * per + (bid >> 3);
-}
-
-template <int TI, int TJ, int TK, bool CC, bool PERSIST>
+// One persistent launch visits every tile.  Workgroup b runs on XCD b % 8 and takes work
+// items (SP_TG consecutive tiles) from that XCD's queue counter (its own 128-byte line,
+// meta[SP_QSTRIDE * (1 + xcd)]) over the XCD's contiguous share of the tiles; the next item
+// is requested before the current one is processed.  Every workgroup leaves when its queue
+// is drained; results do not depend on which workgroup takes which item.  `parts` selects
+// the tiles (1: with candidates, 2: without, 3: all); a split launch does the same work in
+// two launches (used to time the two parts).
+template <int TJ, int TK, bool CC>
 __global__ void __launch_bounds__(SP_THREADS)
     k_spread(GridD g, TileGeom tg, const unsigned* __restrict__ tile_off,
-             unsigned long long* __restrict__ entries, unsigned long long ecap,
-             const double* __restrict__ x, const double* __restrict__ y,
-             const double* __restrict__ z, const double* __restrict__ f1,
-             const double* __restrict__ f2, const double* __restrict__ f3,
-             const double* __restrict__ area, double* __restrict__ o1, double* __restrict__ o2,
-             double* __restrict__ o3) {
-  using T = SpreadTile<TI, TJ, TK, CC>;
+             int* __restrict__ meta, int parts, unsigned long long* __restrict__ entries,
+             unsigned long long ecap, const double* __restrict__ x,
+             const double* __restrict__ y, const double* __restrict__ z,
+             const double* __restrict__ f1, const double* __restrict__ f2,
+             const double* __restrict__ f3, const double* __restrict__ area,
+             double* __restrict__ o1, double* __restrict__ o2, double* __restrict__ o3) {
+  using T = SpreadTile<TJ, TK, CC>;
   __shared__ typename T::Smem sm;
-  if (PERSIST) {
-    // static round-robin over tiles; every workgroup leaves after its last tile
-    for (long long t = blockIdx.x; t < tg.ntiles; t += gridDim.x) {
-      T::run(sm, t, g, tg, tile_off, entries, ecap, x, y, z, f1, f2, f3, area, o1, o2, o3);
-      __syncthreads();
+  const int xcd = blockIdx.x & 7;
+  const long long lo = (long long)xcd * tg.ntiles / 8, hi = (long long)(xcd + 1) * tg.ntiles / 8;
+  const long long nitems = (hi - lo + SP_TG - 1) / SP_TG;
+  int* ctr = meta + SP_QSTRIDE * (1 + xcd);
+  if (threadIdx.x == 0) sm.item = atomicAdd(ctr, 1);
+  __syncthreads();
+  long long item = sm.item;
+  __syncthreads();
+  while (item < nitems) {
+    if (threadIdx.x == 0) sm.item = atomicAdd(ctr, 1);  // request the next item
+    for (int h = 0; h < SP_TG; ++h) {
+      const long long t = lo + item * SP_TG + h;
+      if (t >= hi) break;
+      T::run(sm, t, parts, g, tg, tile_off, entries, ecap, x, y, z, f1, f2, f3, area, o1, o2,
+             o3);
     }
-  } else {
-    long long t = xcd_tile(blockIdx.x, gridDim.x);
-    if (t >= tg.ntiles) return;
-    T::run(sm, t, g, tg, tile_off, entries, ecap, x, y, z, f1, f2, f3, area, o1, o2, o3);
+    __syncthreads();
+    item = sm.item;
+    __syncthreads();
   }
 }
 
 // Wall halo planes k = 0 and k = k1 of lagr2eulr: zero (the scatter never reaches them and
-// updthalos copies only x and y halos, bound.f90:141-190).
+// updthalos copies only x and y halos, bound.f90:141-190).  Two cells per thread.
 __global__ void __launch_bounds__(256) k_zero_kplanes(GridD g, double* __restrict__ o1,
                                                       double* __restrict__ o2,
                                                       double* __restrict__ o3) {
-  long long n = g.sz;  // one plane
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = g.sz;  // cells per plane
+  const long long q = 2 * ((long long)blockIdx.x * blockDim.x + threadIdx.x);
   if (q >= 2 * n) return;
-  long long idx = q < n ? q : q - n + (long long)(g.ktot + 1) * g.sz;
-  o1[idx] = 0.0, o2[idx] = 0.0, o3[idx] = 0.0;
+  const long long idx = q < n ? q : q - n + (long long)(g.ktot + 1) * g.sz;
+  if ((idx & 1) == 0 && q + 1 < 2 * n && (q < n) == (q + 1 < n)) {
+    *reinterpret_cast<double2*>(o1 + idx) = make_double2(0.0, 0.0);
+    *reinterpret_cast<double2*>(o2 + idx) = make_double2(0.0, 0.0);
+    *reinterpret_cast<double2*>(o3 + idx) = make_double2(0.0, 0.0);
+  } else {
+    for (long long e = q; e < q + 2 && e < 2 * n; ++e) {
+      const long long id = e < n ? e : e - n + (long long)(g.ktot + 1) * g.sz;
+      o1[id] = 0.0, o2[id] = 0.0, o3[id] = 0.0;
+    }
+  }
+}
+
+// j halo rows (bound.f90:172-177): var(:,j1,k) = var(:,1,k), var(:,0,k) = var(:,jmax,k) for
+// every plane k in [kfirst, klast], whole rows i = 0..i1 (the tiles computed the i halos).
+__global__ void __launch_bounds__(256) k_jhalo(GridD g, int kfirst, int klast,
+                                               double* __restrict__ o1,
+                                               double* __restrict__ o2,
+                                               double* __restrict__ o3) {
+  const long long ncol = g.sy;
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long per = 2 * ncol;
+  const long long k = kfirst + q / per;
+  if (k > klast) return;
+  const long long rem = q - (k - kfirst) * per;
+  const bool top = rem >= ncol;  // row j1 <- row 1; else row 0 <- row jtot
+  const long long i = top ? rem - ncol : rem;
+  const long long dst = i + (top ? (long long)(g.jtot + 1) : 0) * g.sy + k * g.sz;
+  const long long srcx = i + (top ? 1ll : (long long)g.jtot) * g.sy + k * g.sz;
+  o1[dst] = o1[srcx], o2[dst] = o2[srcx], o3[dst] = o3[srcx];
 }
 
 __global__ void __launch_bounds__(256) k_fill_u32(unsigned* __restrict__ a, long long n,
@@ -557,85 +844,6 @@ __global__ void __launch_bounds__(256) k_fill_u32(unsigned* __restrict__ a, long
 __global__ void __launch_bounds__(64) k_fill_i32(int* __restrict__ a, int n, int v) {
   int q = threadIdx.x;
   if (q < n) a[q] = v;
-}
-
-// ---------------------------------------------------------------------------------------
-// Exclusive scan of the per-tile counts (integers, order-independent).
-// ---------------------------------------------------------------------------------------
-constexpr int SCAN_B = 1024;  // elements per block (256 threads x 4)
-
-__device__ __forceinline__ unsigned block_excl_scan(unsigned v, unsigned* sh, unsigned& total) {
-  // 256 threads
-  const int tid = threadIdx.x;
-  sh[tid] = v;
-  __syncthreads();
-  for (int o = 1; o < 256; o <<= 1) {
-    unsigned t = tid >= o ? sh[tid - o] : 0u;
-    __syncthreads();
-    sh[tid] += t;
-    __syncthreads();
-  }
-  total = sh[255];
-  unsigned incl = sh[tid];
-  __syncthreads();
-  return incl - v;
-}
-
-__global__ void __launch_bounds__(256) k_scan_reduce(const unsigned* __restrict__ in, long long n,
-                                                     unsigned* __restrict__ part) {
-  __shared__ unsigned sh[256];
-  long long b0 = (long long)blockIdx.x * SCAN_B + threadIdx.x * 4;
-  unsigned s = 0;
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-    if (b0 + r < n) s += in[b0 + r];
-  unsigned tot;
-  block_excl_scan(s, sh, tot);
-  if (threadIdx.x == 0) part[blockIdx.x] = tot;
-}
-
-__global__ void __launch_bounds__(256) k_scan_parts(unsigned* __restrict__ part, long long nb) {
-  __shared__ unsigned sh[256];
-  unsigned carry = 0;
-  for (long long c = 0; c < nb; c += 1024) {
-    long long b0 = c + threadIdx.x * 4;
-    unsigned v[4], s = 0;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      v[r] = (b0 + r < nb) ? part[b0 + r] : 0u;
-      s += v[r];
-    }
-    unsigned tot;
-    unsigned ex = block_excl_scan(s, sh, tot) + carry;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      if (b0 + r < nb) part[b0 + r] = ex;
-      ex += v[r];
-    }
-    carry += tot;
-    __syncthreads();
-  }
-}
-
-__global__ void __launch_bounds__(256) k_scan_apply(const unsigned* __restrict__ in, long long n,
-                                                    const unsigned* __restrict__ part,
-                                                    unsigned* __restrict__ out) {
-  __shared__ unsigned sh[256];
-  long long b0 = (long long)blockIdx.x * SCAN_B + threadIdx.x * 4;
-  unsigned v[4], s = 0;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    v[r] = (b0 + r < n) ? in[b0 + r] : 0u;
-    s += v[r];
-  }
-  unsigned tot;
-  unsigned ex = block_excl_scan(s, sh, tot) + part[blockIdx.x];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    if (b0 + r < n) out[b0 + r] = ex;
-    ex += v[r];
-  }
-  if (b0 <= n - 1 && n - 1 < b0 + 4) out[n] = part[blockIdx.x] + tot;  // total
 }
 
 // ---------------------------------------------------------------------------------------

@@ -219,10 +219,14 @@ class Batch:
     def run(self):
         self.ctx._check(self.lib.ibx_batch_run(self.h))
 
+    PROFILE_KEYS = ("interp_us", "bin_us", "spread_us", "total_us", "spread_tiles_only_us",
+                    "spread_zero_only_us", "nonempty_tiles", "tiles", "entries")
+
     def profile(self, reps: int = 10):
-        vals = [ctypes.c_double() for _ in range(4)]
-        self.ctx._check(self.lib.ibx_batch_profile(self.h, int(reps), *[ctypes.byref(v) for v in vals]))
-        return dict(zip(("interp_us", "bin_us", "spread_us", "total_us"), (v.value for v in vals)))
+        """Average stage durations (HIP events, eager launches) and work counts."""
+        vals = (ctypes.c_double * len(self.PROFILE_KEYS))()
+        self.ctx._check(self.lib.ibx_batch_profile(self.h, int(reps), vals))
+        return dict(zip(self.PROFILE_KEYS, list(vals)))
 
     def close(self):
         if getattr(self, "h", None):

@@ -1,0 +1,145 @@
+"""GPU: the HBM-resident entry points and the batched, graph-captured path.
+
+Every tile configuration of the autotuner, eager or replayed from a hipGraph, must give the
+same bits as the CPU oracle: the tile shape changes only which workgroup computes a cell,
+never the order in which the cell receives its terms.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+from suguru_amd import workload as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+
+    assert torch.cuda.is_available(), "no HIP device visible to torch"
+    return torch
+
+
+def bits(a):
+    return np.ascontiguousarray(a).reshape(-1).view(np.uint64)
+
+
+def to_dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).reshape(-1, order="F"))).to("cuda:0")
+
+
+@pytest.fixture(scope="module")
+def moderate():
+    wl = W.config_moderate()
+    g, caps = wl.grid, wl.caps
+    ref_i = O.eulr2lagr(g, wl.u, wl.v, wl.w, caps.cap_off, caps.x, caps.y, caps.z)
+    ref_s = O.lagr2eulr(g, caps.cap_off, caps.x, caps.y, caps.z, wl.fx, wl.fy, wl.fz, caps.area)
+    return wl, ref_i, ref_s
+
+
+def test_device_entry_points_bitexact(gpu_ctx_factory, moderate):
+    torch = _torch()
+    wl, ref_i, ref_s = moderate
+    g, caps = wl.grid, wl.caps
+    print("case: device entry points, 256x128x256", flush=True)
+    ctx = gpu_ctx_factory(g)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    d = {k: to_dev(torch, getattr(wl, k)) for k in ("u", "v", "w", "fx", "fy", "fz")}
+    for k in ("x", "y", "z", "area"):
+        d[k] = to_dev(torch, getattr(caps, k))
+    off = torch.from_numpy(caps.cap_off).to("cuda:0")
+    n = caps.npts
+    ul, vl, wl_ = (torch.empty(n, dtype=torch.float64, device="cuda:0") for _ in range(3))
+    F = [torch.full((g.ncells_halo,), np.nan, dtype=torch.float64, device="cuda:0") for _ in range(3)]
+    ctx.interp_dev(d["u"], d["v"], d["w"], off, d["x"], d["y"], d["z"], ul, vl, wl_)
+    ctx.spread_dev(off, d["x"], d["y"], d["z"], d["fx"], d["fy"], d["fz"], d["area"], *F)
+    ctx.sync()
+    torch.cuda.synchronize()
+    for a, b in zip((ul, vl, wl_), ref_i):
+        assert np.array_equal(bits(a.cpu().numpy()), bits(b))
+    for a, b in zip(F, ref_s):
+        assert np.array_equal(bits(a.cpu().numpy()), bits(b.reshape(-1, order="F")))
+    # the cell-centred spread writes every element too
+    G = [torch.full((g.ncells_halo,), np.nan, dtype=torch.float64, device="cuda:0") for _ in range(3)]
+    ctx.spread_dev(off, d["x"], d["y"], d["z"], d["fx"], d["fy"], d["fz"], d["area"], *G, cc=True)
+    ctx.sync()
+    ref_c = O.lagr2eulrind(g, caps.cap_off, caps.x, caps.y, caps.z, wl.fx, wl.fy, wl.fz, caps.area)
+    for a, b in zip(G, ref_c):
+        assert np.array_equal(bits(a.cpu().numpy()), bits(b.reshape(-1, order="F")))
+    ctx.set_stream(0)
+
+
+def test_batch_every_config_graph_and_eager_bitexact(gpu_ctx_factory, moderate):
+    torch = _torch()
+    from suguru_amd import Batch
+
+    wl, ref_i, ref_s = moderate
+    g, caps = wl.grid, wl.caps
+    ctx = gpu_ctx_factory(g)
+    t = {k: to_dev(torch, getattr(wl, k)) for k in ("u", "v", "w", "fx", "fy", "fz")}
+    for k in ("x", "y", "z", "area"):
+        t[k] = to_dev(torch, getattr(caps, k))
+    n = caps.npts
+    for k in ("ul", "vl", "wl"):
+        t[k] = torch.empty(n, dtype=torch.float64, device="cuda:0")
+    for k in ("forcex", "forcey", "forcez"):
+        t[k] = torch.empty(g.ncells_halo, dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    b = Batch(ctx, caps.cap_off)
+    b.bind(**t)
+    names = Batch.config_names()
+    want_i = [bits(a) for a in ref_i]
+    want_s = [bits(a.reshape(-1, order="F")) for a in ref_s]
+    for cfg, name in enumerate(names):
+        for graph in (True, False):
+            print("case: batch config", name, "graph" if graph else "eager", flush=True)
+            b.set_config(cfg)
+            b.capture(use_graph=graph)
+            for rep in range(3):
+                for k in ("ul", "vl", "wl", "forcex", "forcey", "forcez"):
+                    t[k].fill_(np.nan)  # torch's eager work between replays
+                torch.cuda.synchronize()
+                b.run()
+                ctx.sync()
+                for k, w in zip(("ul", "vl", "wl"), want_i):
+                    assert np.array_equal(bits(t[k].cpu().numpy()), w), (name, graph, rep, k)
+                for k, w in zip(("forcex", "forcey", "forcez"), want_s):
+                    assert np.array_equal(bits(t[k].cpu().numpy()), w), (name, graph, rep, k)
+    prof = b.profile(reps=3)
+    assert prof["total_us"] > 0
+    b.close()
+
+
+def test_batch_replays_follow_moving_points(gpu_ctx_factory):
+    """The graph reads the bound buffers at replay time: after Update_Pos moves the points in
+    place, the next replay equals the oracle at the new positions."""
+    torch = _torch()
+    from suguru_amd import Batch
+
+    wl = W.config_small()
+    g, caps = wl.grid, wl.caps
+    ctx = gpu_ctx_factory(g)
+    t = {k: to_dev(torch, getattr(wl, k)) for k in ("u", "v", "w", "fx", "fy", "fz")}
+    for k in ("x", "y", "z", "area"):
+        t[k] = to_dev(torch, getattr(caps, k))
+    for k in ("ul", "vl", "wl"):
+        t[k] = torch.empty(caps.npts, dtype=torch.float64, device="cuda:0")
+    for k in ("forcex", "forcey", "forcez"):
+        t[k] = torch.empty(g.ncells_halo, dtype=torch.float64, device="cuda:0")
+    off = torch.from_numpy(caps.cap_off).to("cuda:0")
+    cen = torch.empty((caps.ncap, 3), dtype=torch.float64, device="cuda:0")
+    um = torch.empty((caps.ncap, 3), dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    b = Batch(ctx, caps.cap_off)
+    b.bind(**t)
+    b.capture(use_graph=True)
+    x, y, z = caps.x.copy(), caps.y.copy(), caps.z.copy()
+    for step in range(4):
+        b.run()
+        ctx.update_pos_dev(off, t["x"], t["y"], t["z"], t["ul"], t["vl"], t["wl"], 2e-3, cen, um)
+        ctx.sync()
+        ri = O.eulr2lagr(g, wl.u, wl.v, wl.w, caps.cap_off, x, y, z)
+        O.update_pos(g, caps.cap_off, 2e-3, x, y, z, *[np.array(a) for a in ri])
+        for k, a in zip("xyz", (x, y, z)):
+            assert np.array_equal(bits(t[k].cpu().numpy()), bits(a)), (step, k)
+    b.close()

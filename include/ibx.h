@@ -180,11 +180,22 @@ const char* ibx_batch_config_name(int choice);
 /* Capture interp + spread on the context stream into a graph (use_graph = 1) or run eagerly
  * (use_graph = 0) on every ibx_batch_run. */
 int ibx_batch_capture(ibx_batch* b, int use_graph);
+/* Capture `nslots` graph instances of the same step, each of which also records HIP events
+ * (on the context stream) at the start, after the interpolation, before and after the tile
+ * spread kernel, and at the end.  ibx_batch_run then replays instance (run count % nslots),
+ * so nslots consecutive runs leave one timing record each. */
+#define IBX_TIMED_EV 5
+int ibx_batch_capture_timed(ibx_batch* b, int nslots);
 int ibx_batch_run(ibx_batch* b);
+/* Wait for the stream and restart the run count at slot 0. */
+int ibx_batch_reset_slots(ibx_batch* b);
+/* Wait for the stream; us[4*s + i] receives slot s's last replay, in microseconds:
+ * [0] interpolation, [1] binning + scan + wall planes, [2] tile spread kernel, [3] whole step. */
+int ibx_batch_slot_times(ibx_batch* b, double* us);
 /* Run interp + spread `reps` times eagerly with HIP events on the context stream around each
  * stage; us[0..IBX_PROF_N) receives, in microseconds: [0] interpolation kernel, [1] binning
  * (count, scan, fill), [2] tile spread kernel, [3] whole sequence, [4] tile spread kernel's
- * non-empty tiles and [5] its zero rows when run as two launches; and the work counts of the
+ * tiles with candidates and [5] its tiles without candidates when run as two launches; and the work counts of the
  * last run: [6] non-empty tiles, [7] tiles, [8] candidate entries. */
 #define IBX_PROF_N 9
 int ibx_batch_profile(ibx_batch* b, int reps, double* us);

@@ -101,7 +101,10 @@ def load() -> ctypes.CDLL:
         "ibx_batch_num_configs": (ctypes.c_int, []),
         "ibx_batch_config_name": (ctypes.c_char_p, [ctypes.c_int]),
         "ibx_batch_capture": (ctypes.c_int, [P, ctypes.c_int]),
+        "ibx_batch_capture_timed": (ctypes.c_int, [P, ctypes.c_int]),
         "ibx_batch_run": (ctypes.c_int, [P]),
+        "ibx_batch_reset_slots": (ctypes.c_int, [P]),
+        "ibx_batch_slot_times": (ctypes.c_int, [P, ctypes.POINTER(D)]),
         "ibx_batch_profile": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(D)]),
         "ibx_batch_destroy": (None, [P]),
     }

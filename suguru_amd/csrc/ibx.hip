@@ -53,7 +53,7 @@ const TileCfg kCfgs[] = {
     IBX_CFG(16, 4, 8, "chunk16x16x4"),
     IBX_CFG(8, 8, 4, "chunk16x8x8-w4"),
     IBX_CFG(8, 4, 12, "chunk16x8x4-w12"),
-    IBX_CFG(16, 8, 6, "chunk16x16x8"),
+    IBX_CFG(8, 4, 16, "chunk16x8x4-w16"),
     IBX_CFG(4, 16, 8, "chunk16x4x16"),
 };
 constexpr int kNumCfgs = (int)(sizeof(kCfgs) / sizeof(kCfgs[0]));
@@ -88,6 +88,12 @@ struct ibx_batch {
   hipGraphExec_t exec = nullptr;
   bool use_graph = false;
   bool captured = false;
+  // timed slots: graph instances that also record IBX_TIMED_EV events each; run n replays
+  // slot n % nslots
+  std::vector<hipGraph_t> tgraphs;
+  std::vector<hipGraphExec_t> texecs;
+  std::vector<hipEvent_t> tev;
+  long long runs = 0;
 };
 
 namespace {
@@ -181,7 +187,8 @@ int enqueue_spread(ibx_ctx* c, int cfgid, bool cc, long long ncap, const long lo
                    long long npts, const double* x, const double* y, const double* z,
                    const double* f1, const double* f2, const double* f3, const double* area,
                    double* o1, double* o2, double* o3, bool size_entries,
-                   hipEvent_t ev_tile = nullptr, hipEvent_t ev_split = nullptr) {
+                   hipEvent_t ev_tile = nullptr, hipEvent_t ev_split = nullptr,
+                   hipEvent_t ev_tile_end = nullptr) {
   const TileCfg& cf = kCfgs[cfgid];
   TileGeom tg = geom(c->grid, cf, cc);
   int rc;
@@ -215,13 +222,13 @@ int enqueue_spread(ibx_ctx* c, int cfgid, bool cc, long long ncap, const long lo
     k_bin<1><<<blocks_for(npts, 256), 256, 0, c->stream>>>(
         c->gd, tg, ncap, d_off, npts, x, y, z, cnt, off, (unsigned long long*)c->entries.p,
         c->ecap, c->d_err);
-  if (ev_tile) HIPCHK(c, hipEventRecord(ev_tile, c->stream));
   SpreadFn fn = cc ? cf.cc : cf.plain;
   const unsigned nblk = (unsigned)((long long)c->num_cus * cf.wg_per_cu);
   if (!cc) {
     const long long n2 = 2 * c->gd.sz;
     k_zero_kplanes<<<blocks_for((n2 + 1) / 2, 256), 256, 0, c->stream>>>(c->gd, o1, o2, o3);
   }
+  if (ev_tile) HIPCHK(c, hipEventRecord(ev_tile, c->stream));
   if (!ev_split) {
     fn<<<nblk, SP_THREADS, 0, c->stream>>>(c->gd, tg, off, (int*)c->tmeta.p, 3,
                                             (unsigned long long*)c->entries.p, c->ecap, x, y, z,
@@ -238,6 +245,7 @@ int enqueue_spread(ibx_ctx* c, int cfgid, bool cc, long long ncap, const long lo
                                             (unsigned long long*)c->entries.p, c->ecap, x, y, z,
                                             f1, f2, f3, area, o1, o2, o3);
   }
+  if (ev_tile_end) HIPCHK(c, hipEventRecord(ev_tile_end, c->stream));
   {
     const int kf = tg.kbase, kl = tg.kbase + tg.nkp - 1;
     const long long nj = (long long)(kl - kf + 1) * 2 * c->gd.sy;
@@ -616,17 +624,29 @@ int ibx_batch_bind(ibx_batch* b, const ibx_batch_bufs* bufs) {
   return IBX_OK;
 }
 
-static int batch_enqueue(ibx_batch* b, bool size_entries) {
+// ev (optional): IBX_TIMED_EV events recorded at the start, after the interpolation, before
+// and after the tile spread kernel, and at the end
+static int batch_enqueue(ibx_batch* b, bool size_entries, const hipEvent_t* ev = nullptr) {
   ibx_ctx* c = b->ctx;
   const ibx_batch_bufs& q = b->bufs;
+  if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
   int rc = enqueue_interp(c, q.u, q.v, q.w, b->npts, q.x, q.y, q.z, q.ul, q.vl, q.wl);
   if (rc) return rc;
-  return enqueue_spread(c, b->cfg, false, b->ncap, (const long long*)b->d_off.p, b->npts, q.x,
-                        q.y, q.z, q.fx, q.fy, q.fz, q.area, q.forcex, q.forcey, q.forcez,
-                        size_entries);
+  if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+  rc = enqueue_spread(c, b->cfg, false, b->ncap, (const long long*)b->d_off.p, b->npts, q.x,
+                      q.y, q.z, q.fx, q.fy, q.fz, q.area, q.forcex, q.forcey, q.forcez,
+                      size_entries, ev ? ev[2] : nullptr, nullptr, ev ? ev[3] : nullptr);
+  if (rc) return rc;
+  if (ev) HIPCHK(c, hipEventRecord(ev[4], c->stream));
+  return IBX_OK;
 }
 
 static void drop_graph(ibx_batch* b) {
+  for (auto e : b->texecs) (void)hipGraphExecDestroy(e);
+  for (auto g : b->tgraphs) (void)hipGraphDestroy(g);
+  for (auto e : b->tev) (void)hipEventDestroy(e);
+  b->texecs.clear(), b->tgraphs.clear(), b->tev.clear();
+  b->runs = 0;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   b->exec = nullptr;
@@ -641,12 +661,31 @@ int ibx_batch_set_config(ibx_batch* b, int choice) {
   return IBX_OK;
 }
 
-int ibx_batch_capture(ibx_batch* b, int use_graph) {
+static int capture_graph(ibx_batch* b, const hipEvent_t* ev, hipGraph_t* graph,
+                         hipGraphExec_t* exec) {
+  ibx_ctx* c = b->ctx;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = batch_enqueue(b, false, ev);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(c, IBX_E_HIP, std::string("capture: ") + hipGetErrorString(e));
+  *graph = g;
+  HIPCHK(c, hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+  return IBX_OK;
+}
+
+static int capture_common(ibx_batch* b, int use_graph, int nslots) {
   if (!b) return IBX_E_ARG;
   ibx_ctx* c = b->ctx;
   if (!b->bound) return fail(c, IBX_E_STATE, "batch not bound");
+  if (nslots < 0 || nslots > 4096) return fail(c, IBX_E_ARG, "nslots out of range 0..4096");
   HIPCHK(c, hipSetDevice(c->device));
   drop_graph(b);
+  b->captured = false;
   // one eager pass sizes every work buffer; the candidate list then gets room for the points
   // to move: 1.5x the measured count, and at least 4 entries per point (a point's window
   // lists it in ~2-3 tiles on average)
@@ -665,22 +704,29 @@ int ibx_batch_capture(ibx_batch* b, int use_graph) {
       c->ecap = c->entries.bytes / sizeof(unsigned long long);
     }
   }
-  b->use_graph = use_graph != 0;
-  if (b->use_graph) {
-    HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    rc = batch_enqueue(b, false);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (rc) {
-      if (g) (void)hipGraphDestroy(g);
-      return rc;
-    }
-    if (e != hipSuccess) return fail(c, IBX_E_HIP, std::string("capture: ") + hipGetErrorString(e));
-    b->graph = g;
-    HIPCHK(c, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  b->use_graph = use_graph != 0 || nslots > 0;
+  if (nslots > 0) {
+    b->tev.assign((size_t)nslots * IBX_TIMED_EV, nullptr);
+    for (auto& e : b->tev) HIPCHK(c, hipEventCreate(&e));
+    b->tgraphs.assign(nslots, nullptr);
+    b->texecs.assign(nslots, nullptr);
+    for (int sl = 0; sl < nslots; ++sl)
+      if ((rc = capture_graph(b, &b->tev[(size_t)sl * IBX_TIMED_EV], &b->tgraphs[sl],
+                              &b->texecs[sl])))
+        return rc;
+  } else if (b->use_graph) {
+    if ((rc = capture_graph(b, nullptr, &b->graph, &b->exec))) return rc;
   }
+  b->runs = 0;
   b->captured = true;
   return IBX_OK;
+}
+
+int ibx_batch_capture(ibx_batch* b, int use_graph) { return capture_common(b, use_graph, 0); }
+
+int ibx_batch_capture_timed(ibx_batch* b, int nslots) {
+  if (nslots <= 0) return b ? fail(b->ctx, IBX_E_ARG, "nslots must be positive") : IBX_E_ARG;
+  return capture_common(b, 1, nslots);
 }
 
 int ibx_batch_run(ibx_batch* b) {
@@ -688,11 +734,46 @@ int ibx_batch_run(ibx_batch* b) {
   ibx_ctx* c = b->ctx;
   if (!b->captured) return fail(c, IBX_E_STATE, "batch not captured");
   HIPCHK(c, hipSetDevice(c->device));
+  if (!b->texecs.empty()) {
+    const size_t sl = (size_t)(b->runs % (long long)b->texecs.size());
+    HIPCHK(c, hipGraphLaunch(b->texecs[sl], c->stream));
+    ++b->runs;
+    return IBX_OK;
+  }
+  ++b->runs;
   if (b->use_graph) {
     HIPCHK(c, hipGraphLaunch(b->exec, c->stream));
     return IBX_OK;
   }
   return batch_enqueue(b, false);
+}
+
+int ibx_batch_reset_slots(ibx_batch* b) {
+  if (!b) return IBX_E_ARG;
+  ibx_ctx* c = b->ctx;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  b->runs = 0;
+  return IBX_OK;
+}
+
+int ibx_batch_slot_times(ibx_batch* b, double* us) {
+  if (!b || !us) return IBX_E_ARG;
+  ibx_ctx* c = b->ctx;
+  if (b->texecs.empty()) return fail(c, IBX_E_STATE, "batch not captured with timed slots");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const size_t ns = b->texecs.size();
+  for (size_t sl = 0; sl < ns; ++sl) {
+    const hipEvent_t* e = &b->tev[sl * IBX_TIMED_EV];
+    float t[4];
+    HIPCHK(c, hipEventElapsedTime(&t[0], e[0], e[1]));  // interpolation
+    HIPCHK(c, hipEventElapsedTime(&t[1], e[1], e[2]));  // binning, scan, wall planes
+    HIPCHK(c, hipEventElapsedTime(&t[2], e[2], e[3]));  // tile spread kernel
+    HIPCHK(c, hipEventElapsedTime(&t[3], e[0], e[4]));  // whole step
+    for (int i = 0; i < 4; ++i) us[sl * 4 + i] = 1000.0 * (double)t[i];
+  }
+  return collect_errors(c);
 }
 
 int ibx_batch_profile(ibx_batch* b, int reps, double* us) {

@@ -536,25 +536,26 @@ __device__ void bitonic_sort(unsigned long long* a, int n, int tid) {
 
 template <int TJ, int TK, bool CC>
 struct SpreadTile {
-  static constexpr int NR = TJ * TK;         // rows per tile
-  static constexpr int NC = SP_CW * NR;      // cells per tile
-  static constexpr int NP = NC / 2;          // cell pairs per tile
-  static constexpr int PPT = NP / SP_THREADS;  // pairs per thread
+  static constexpr int NR = TJ * TK;           // rows per tile
+  static constexpr int NC = SP_CW * NR;        // cells per tile
+  static constexpr int NP = NC / 2;            // cell pairs per tile
+  static constexpr int PPT = NP / SP_THREADS;  // pairs per thread in the write
+  static constexpr int CPT = NC / SP_THREADS;  // cells per thread in the per-cell passes
   static_assert(NP % SP_THREADS == 0, "tile must hold a multiple of 512 cells");
   static_assert(TJ >= 4 && TK >= 4 && TJ <= 64 && TK <= 64, "row block 4..64 each way");
 
   struct Smem {
     unsigned long long keys[SP_CAPK];
     unsigned long long mask[NC];
-    double2 wp[SP_CH][3][4];  // per candidate, direction, tap: (centred, staggered) weight
-    double2 fa[SP_CH][2];     // (f1, f2), (f3, area/dVeul)
-    int wx0[SP_CH];           // x window origin (virtual i of its first tap)
-    int xr[SP_CH];            // clamped x window: a | b << 16
-    union {
-      int w;
-      unsigned char b[4];
-    } org[SP_CH];             // y, z window origin relative to the tile (bytes 1, 2), +8
-    int box[SP_CH][2];        // clamped y, z window within the tile: a | b << 8
+    double acc[3][NC];          // running sums of the tile's cells, per force component
+    unsigned short order[NC];   // cells with set bits, by decreasing number of bits
+    int hist[SP_CH + 1];        // per bit count: cells, then scatter cursors
+    int nz;                     // cells with set bits in this chunk
+    double4 fw[SP_CH][4];       // per candidate and x tap: f1*wx, f2*wx, f3*wx, area/dVeul
+    double2 wyz[SP_CH][2][4];   // per candidate, y / z tap: (centred, staggered) weight
+    int org[SP_CH];             // window origin: x (bits 0-15, +2^15), y, z rel. tile (+8)
+    int xr[SP_CH];              // clamped x window: a | b << 16
+    int box[SP_CH][2];          // clamped y, z window within the tile: a | b << 8
     int item;
   };
 
@@ -594,12 +595,12 @@ struct SpreadTile {
     const int n = (unsigned long long)base < endl ? (int)(endl - base) : 0;
     if (!(parts & (n > 0 ? 1 : 2))) return;
 
-    // this thread's cells: pairs tid + h*256, cells 2*pair and 2*pair+1
-    double acc1[2 * PPT], acc2[2 * PPT], acc3[2 * PPT];
-#pragma unroll
-    for (int r = 0; r < 2 * PPT; ++r) acc1[r] = acc2[r] = acc3[r] = 0.0;
-
     if (n > 0) {
+#pragma unroll
+      for (int r = 0; r < CPT; ++r)
+        sm.acc[0][tid + r * SP_THREADS] = sm.acc[1][tid + r * SP_THREADS] =
+            sm.acc[2][tid + r * SP_THREADS] = 0.0;
+      // (a) K-sort of the tile's candidates
       const unsigned long long* src;
       if (n <= SP_THREADS) {
         // rank sort: keys are unique, rank = number of smaller keys
@@ -626,55 +627,64 @@ struct SpreadTile {
       for (int c0 = 0; c0 < n; c0 += SP_CH) {
         const int m = (n - c0) < SP_CH ? (n - c0) : SP_CH;
 #pragma unroll
-        for (int r = 0; r < NC / SP_THREADS; ++r) sm.mask[tid + r * SP_THREADS] = 0ull;
-        // (b) candidate preparation: 4 threads per candidate (x, y, z weights; values)
+        for (int r = 0; r < CPT; ++r) sm.mask[tid + r * SP_THREADS] = 0ull;
+        if (tid <= SP_CH) sm.hist[tid] = 0;
+        // (b) candidate preparation, 4 threads per candidate: x weights times the values,
+        // y weights, z weights
         for (int c = tid >> 2; c < m; c += SP_THREADS / 4) {
           const int d = tid & 3;
+          if (d == 3) continue;
           const unsigned long long e = src[c0 + c];
           const unsigned lo32 = (unsigned)(e & 0xffffffffull);
           const long long q = (long long)(lo32 & 0x0fffffffu);
           const int s = nb_slot((int)(lo32 >> 28));
-          if (d < 3) {
-            double cc_;
-            int lo, hi, base_, ext;
-            if (d == 0) {
-              cc_ = shifted_coord(x[q], slot_xs(s), g.lx, g.bl, g.dxi);
-              lo = 1, hi = g.itot, base_ = 0, ext = 1 << 30;
-            } else if (d == 1) {
-              cc_ = shifted_coord(y[q], slot_ys(s), g.ly, g.bf, g.dyi);
-              lo = 1, hi = g.jtot, base_ = j0, ext = TJ;
-            } else {
-              cc_ = z[q] * g.dzi;
-              lo = tg.klo, hi = tg.khi, base_ = k0, ext = TK;
-            }
-            Axis ax;
-            int w0 = 0, bl = 1, bh = 0;
-            if (axis_of(cc_, lo, hi, ax)) {
-              const bool n0 = ax.na == 0;
-              sm.wp[c][d][0] = make_double2(n0 ? ax.ns[0] : 0.0, 0.0);
-              sm.wp[c][d][1] = make_double2(n0 ? ax.ns[1] : ax.ns[0], ax.st[0]);
-              sm.wp[c][d][2] = make_double2(n0 ? ax.ns[2] : ax.ns[1], ax.st[1]);
-              sm.wp[c][d][3] = make_double2(n0 ? 0.0 : ax.ns[2], ax.st[2]);
-              int a, b;
-              window(cc_, lo, hi, w0, a, b);
-              w0 -= base_;
-              bl = max(a - base_, 0);
-              bh = min(b - base_, ext - 1);
-            }
-            if (d == 0) {
-              sm.wx0[c] = w0;
-              sm.xr[c] = bl | (bh << 16);
-            } else {
-              sm.box[c][d - 1] = bl | (bh << 8);
-              sm.org[c].b[d] = (unsigned char)(w0 + 8);
-            }
+          double cc_;
+          int lo, hi, base_, ext;
+          if (d == 0) {
+            cc_ = shifted_coord(x[q], slot_xs(s), g.lx, g.bl, g.dxi);
+            lo = 1, hi = g.itot, base_ = 0, ext = 1 << 30;
+          } else if (d == 1) {
+            cc_ = shifted_coord(y[q], slot_ys(s), g.ly, g.bf, g.dyi);
+            lo = 1, hi = g.jtot, base_ = j0, ext = TJ;
           } else {
-            sm.fa[c][0] = make_double2(f1[q], f2[q]);
-            sm.fa[c][1] = make_double2(f3[q], area[q] / g.dveul);
+            cc_ = z[q] * g.dzi;
+            lo = tg.klo, hi = tg.khi, base_ = k0, ext = TK;
+          }
+          Axis ax;
+          int w0 = 0, bl = 1, bh = 0;
+          double wn[4] = {0.0, 0.0, 0.0, 0.0}, ws[4] = {0.0, 0.0, 0.0, 0.0};
+          if (axis_of(cc_, lo, hi, ax)) {
+            const bool n0 = ax.na == 0;
+            wn[0] = n0 ? ax.ns[0] : 0.0;
+            wn[1] = n0 ? ax.ns[1] : ax.ns[0];
+            wn[2] = n0 ? ax.ns[2] : ax.ns[1];
+            wn[3] = n0 ? 0.0 : ax.ns[2];
+            ws[1] = ax.st[0], ws[2] = ax.st[1], ws[3] = ax.st[2];
+            int a, b;
+            window(cc_, lo, hi, w0, a, b);
+            w0 -= base_;
+            bl = max(a - base_, 0);
+            bh = min(b - base_, ext - 1);
+          }
+          if (d == 0) {
+            const double v1 = f1[q], v2 = f2[q], v3 = f3[q];
+            const double adv = area[q] / g.dveul;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              sm.fw[c][u] = CC ? make_double4(v1 * ws[u], v2 * ws[u], v3 * ws[u], adv)
+                               : make_double4(v1 * wn[u], v2 * ws[u], v3 * ws[u], adv);
+            sm.xr[c] = bl | (bh << 16);
+            // the origin's bytes are written separately by the x, y and z threads
+            reinterpret_cast<unsigned short*>(&sm.org[c])[0] = (unsigned short)(w0 + 32768);
+          } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) sm.wyz[c][d - 1][u] = make_double2(wn[u], ws[u]);
+            sm.box[c][d - 1] = bl | (bh << 8);
+            reinterpret_cast<unsigned char*>(&sm.org[c])[d + 1] = (unsigned char)(w0 + 8);
           }
         }
         __syncthreads();
-        // mark the cells each candidate touches: 4 threads per candidate, one x-tap each,
+        // (c) mark the cells each candidate touches: 4 threads per candidate, one x tap each,
         // over the window's rows within the tile; a tap at i = itot also marks the halo
         // duplicate at i = 0, a tap at i = 1 the one at i = itot+1
         for (int c = tid >> 2; c < m; c += SP_THREADS / 4) {
@@ -702,39 +712,68 @@ struct SpreadTile {
           }
         }
         __syncthreads();
-        // (c) ordered accumulation: set bits in ascending order = K order
+        // (d) order the marked cells by decreasing bit count (a counting sort), so that the
+        // lanes of a wavefront get cells with about equal numbers of terms
+        int pc[CPT];
 #pragma unroll
-        for (int r = 0; r < 2 * PPT; ++r) {
-          const int cell = 2 * (tid + (r >> 1) * SP_THREADS) + (r & 1);
+        for (int r = 0; r < CPT; ++r) {
+          pc[r] = __popcll(sm.mask[tid + r * SP_THREADS]);
+          if (pc[r]) atomicAdd(&sm.hist[SP_CH - pc[r]], 1);
+        }
+        __syncthreads();
+        if (tid < 64) {  // exclusive scan of hist[0..SP_CH) by one wavefront
+          const int v = tid < SP_CH ? sm.hist[tid] : 0;
+          int incl = v;
+#pragma unroll
+          for (int o = 1; o < 64; o <<= 1) {
+            const int u = __shfl_up(incl, o);
+            if (tid >= o) incl += u;
+          }
+          if (tid < SP_CH) sm.hist[tid] = incl - v;
+          if (tid == 63) sm.nz = incl;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < CPT; ++r)
+          if (pc[r])
+            sm.order[atomicAdd(&sm.hist[SP_CH - pc[r]], 1)] = (unsigned short)(tid + r * SP_THREADS);
+        __syncthreads();
+        // (e) ordered accumulation: each cell adds its terms in ascending bit order, i.e. in
+        // K order: cell (i,j,k) receives the reference's sequence
+        // force = force + f*wx*wy*wz*(area/dVeul) (:869-874 / :1334-1338)
+        const int nz = sm.nz;
+        for (int rk = tid; rk < nz; rk += SP_THREADS) {
+          const int cell = sm.order[rk];
           unsigned long long mk = sm.mask[cell];
-          if (!mk) continue;
           int i, vi, lj, lk;
           long long mm;
           cell_of(g, tg, col, j0, k0, cell, i, vi, lj, lk, mm);
+          double a1 = sm.acc[0][cell], a2 = sm.acc[1][cell], a3 = sm.acc[2][cell];
+          const int ox = vi + 32768, oy = lj + 8, oz = lk + 8;
           while (mk) {
             const int c = __builtin_ctzll(mk);
             mk &= mk - 1;
-            const int o = sm.org[c].w;
-            const int a = vi - sm.wx0[c], b = lj - (((o >> 8) & 0xff) - 8),
-                      d = lk - (((o >> 16) & 0xff) - 8);
-            const double2 wxp = sm.wp[c][0][a], wyp = sm.wp[c][1][b], wzp = sm.wp[c][2][d];
-            const double2 fa0 = sm.fa[c][0], fa1 = sm.fa[c][1];
-            const double adv = fa1.y;
+            const unsigned o = (unsigned)sm.org[c];
+            const int a = ox - (int)(o & 0xffff), b = oy - (int)((o >> 16) & 0xff),
+                      d = oz - (int)(o >> 24);
+            const double4 fw = sm.fw[c][a];
+            const double2 wy = sm.wyz[c][0][b], wz = sm.wyz[c][1][d];
             if (!CC) {
-              acc1[r] = acc1[r] + fa0.x * wxp.x * wyp.y * wzp.y * adv;
-              acc2[r] = acc2[r] + fa0.y * wxp.y * wyp.x * wzp.y * adv;
-              acc3[r] = acc3[r] + fa1.x * wxp.y * wyp.y * wzp.x * adv;
+              a1 = a1 + fw.x * wy.y * wz.y * fw.w;
+              a2 = a2 + fw.y * wy.x * wz.y * fw.w;
+              a3 = a3 + fw.z * wy.y * wz.x * fw.w;
             } else {
-              acc1[r] = acc1[r] + fa0.x * wxp.y * wyp.y * wzp.y * adv;
-              acc2[r] = acc2[r] + fa0.y * wxp.y * wyp.y * wzp.y * adv;
-              acc3[r] = acc3[r] + fa1.x * wxp.y * wyp.y * wzp.y * adv;
+              a1 = a1 + fw.x * wy.y * wz.y * fw.w;
+              a2 = a2 + fw.y * wy.y * wz.y * fw.w;
+              a3 = a3 + fw.z * wy.y * wz.y * fw.w;
             }
           }
+          sm.acc[0][cell] = a1, sm.acc[1][cell] = a2, sm.acc[2][cell] = a3;
         }
         __syncthreads();
       }
     }
-    // (d) write: each thread's pairs are two adjacent positions of one row, 16 bytes
+    // (f) write: each thread's pairs are two adjacent positions of one row, 16 bytes
 #pragma unroll
     for (int h = 0; h < PPT; ++h) {
       const int cell = 2 * (tid + h * SP_THREADS);
@@ -742,16 +781,22 @@ struct SpreadTile {
       long long m0, m1;
       const bool v0 = cell_of(g, tg, col, j0, k0, cell, i0, vi0, lj, lk, m0);
       const bool v1 = cell_of(g, tg, col, j0, k0, cell + 1, i1, vi1, lj1, lk1, m1);
-      const int r0 = 2 * h, r1 = 2 * h + 1;
+      double2 r1 = make_double2(0.0, 0.0), r2 = r1, r3 = r1;
+      if (n > 0) {
+        r1 = *reinterpret_cast<const double2*>(&sm.acc[0][cell]);
+        r2 = *reinterpret_cast<const double2*>(&sm.acc[1][cell]);
+        r3 = *reinterpret_cast<const double2*>(&sm.acc[2][cell]);
+      }
       if (v0 && v1 && (m0 & 1) == 0) {
-        *reinterpret_cast<double2*>(o1 + m0) = make_double2(acc1[r0], acc1[r1]);
-        *reinterpret_cast<double2*>(o2 + m0) = make_double2(acc2[r0], acc2[r1]);
-        *reinterpret_cast<double2*>(o3 + m0) = make_double2(acc3[r0], acc3[r1]);
+        *reinterpret_cast<double2*>(o1 + m0) = r1;
+        *reinterpret_cast<double2*>(o2 + m0) = r2;
+        *reinterpret_cast<double2*>(o3 + m0) = r3;
       } else {
-        if (v0) o1[m0] = acc1[r0], o2[m0] = acc2[r0], o3[m0] = acc3[r0];
-        if (v1) o1[m1] = acc1[r1], o2[m1] = acc2[r1], o3[m1] = acc3[r1];
+        if (v0) o1[m0] = r1.x, o2[m0] = r2.x, o3[m0] = r3.x;
+        if (v1) o1[m1] = r1.y, o2[m1] = r2.y, o3[m1] = r3.y;
       }
     }
+    if (n > 0) __syncthreads();  // the next tile reuses the accumulators
   }
 };
 

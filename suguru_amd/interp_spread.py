@@ -216,8 +216,27 @@ class Batch:
     def capture(self, use_graph: bool = True):
         self.ctx._check(self.lib.ibx_batch_capture(self.h, 1 if use_graph else 0))
 
+    def capture_timed(self, nslots: int):
+        """Capture `nslots` graph instances that also record HIP events around each stage;
+        run() cycles through them (ibx_batch_capture_timed)."""
+        self.ctx._check(self.lib.ibx_batch_capture_timed(self.h, int(nslots)))
+        self._nslots = int(nslots)
+
     def run(self):
         self.ctx._check(self.lib.ibx_batch_run(self.h))
+
+    def reset_slots(self):
+        self.ctx._check(self.lib.ibx_batch_reset_slots(self.h))
+
+    SLOT_KEYS = ("interp_us", "bin_us", "tile_spread_us", "step_us")
+
+    def slot_times(self):
+        """Per timed slot (its last replay): stage durations in microseconds."""
+        n = getattr(self, "_nslots", 0)
+        vals = (ctypes.c_double * (4 * n))()
+        self.ctx._check(self.lib.ibx_batch_slot_times(self.h, vals))
+        v = list(vals)
+        return [dict(zip(self.SLOT_KEYS, v[4 * i:4 * i + 4])) for i in range(n)]
 
     PROFILE_KEYS = ("interp_us", "bin_us", "spread_us", "total_us", "spread_tiles_only_us",
                     "spread_zero_only_us", "nonempty_tiles", "tiles", "entries")

@@ -107,6 +107,23 @@ def test_batch_every_config_graph_and_eager_bitexact(gpu_ctx_factory, moderate):
                     assert np.array_equal(bits(t[k].cpu().numpy()), w), (name, graph, rep, k)
     prof = b.profile(reps=3)
     assert prof["total_us"] > 0
+    # timed graph instances (the benchmark's timed region): same bits, one timing record each
+    print("case: timed slots", flush=True)
+    b.set_config(0)
+    b.capture_timed(3)
+    b.reset_slots()
+    for rep in range(3):
+        for k in ("forcex", "forcey", "forcez"):
+            t[k].fill_(np.nan)
+        torch.cuda.synchronize()
+        b.run()
+        ctx.sync()
+        for k, w in zip(("forcex", "forcey", "forcez"), want_s):
+            assert np.array_equal(bits(t[k].cpu().numpy()), w), ("timed", rep, k)
+    times = b.slot_times()
+    assert len(times) == 3
+    for s in times:
+        assert 0 < s["tile_spread_us"] < s["step_us"] and s["interp_us"] > 0
     b.close()
 
 

@@ -1,0 +1,117 @@
+/*
+ * memb.h -- C ABI of the capsule-membrane library (libmemb.so, gfx950).
+ *
+ * Two ways to get the membrane load of every capsule:
+ *
+ *  1. The FP64 spherical-harmonic membrane law of the reference (neo-Hookean shear plus
+ *     bending about the reference curvature, dealiased on the 2x grid):
+ *       complagrforces      forcing.f90:11-114   (force grid = harmonic grid: TRSSPH is the
+ *                                                 identity)
+ *       cal_spectral_force  sph.f90:1523-1755
+ *     -> memb_traction_f64_dev, one workgroup per capsule, every capsule in one launch.
+ *
+ *  2. A learned surrogate of that law: an MLP from the capsule's real spherical-harmonic
+ *     shape coefficients to the traction at its points, run and trained with hand-written
+ *     BF16 MFMA kernels that accumulate in FP32 (memb_mlp_*).  It has no counterpart in the
+ *     reference; the FP64 law is its training-label source and its fallback.
+ *
+ * Capsule geometry: degree L, (L+1) Gauss-Legendre colatitudes (increasing) x 2(L+1) uniform
+ * longitudes phi_j = 2 pi j / (2(L+1)); point l = i * 2(L+1) + j (latitude-major,
+ * forcing.f90:37-43); positions x = xc + R sin(t) sin(p), y = yc + R sin(t) cos(p),
+ * z = zc + R cos(t) for a sphere (sph.f90:84-86).  The reference (stress-free) state is the
+ * unit sphere (sph.f90:56-70).  Point arrays are capsule-major: x[c * npts + l].
+ *
+ * Shape coefficients (SPHEREPACK convention, sph1.f90:29-79):
+ *   f = sum_{m<=n<=L} Pbar_n^m(cos t) (a_mn cos m p - b_mn sin m p), m = 0 terms halved,
+ * packed per coordinate as a_mn for m = 0..L, n = m..L, then b_mn for m = 1..L, n = m..L:
+ * (L+1)^2 values per coordinate, 3 (L+1)^2 per capsule (x block, y block, z block).
+ *
+ * Errors: IBX-style negative codes (0 = OK) and memb_last_error().
+ */
+#ifndef MEMB_H
+#define MEMB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MEMB_OK 0
+#define MEMB_E_ARG (-1)
+#define MEMB_E_HIP (-2)
+#define MEMB_E_STATE (-5)
+
+typedef struct memb_plan memb_plan;
+
+/* Tables (Gauss grid, Legendre functions and their theta-derivatives up to 4th order, the
+ * unit-sphere reference state on the 2x grid) for degree L (2 <= L <= 23) on HIP device
+ * `device`.  Replaces capsule_init_cal_coeff (sph.f90:16-191). */
+int memb_plan_create(memb_plan** out, int L, int device);
+void memb_plan_destroy(memb_plan* p);
+const char* memb_last_error(const memb_plan* p);
+int memb_set_stream(memb_plan* p, void* hip_stream);
+int memb_sync(memb_plan* p);
+/* points per capsule (L+1) * 2(L+1), and coefficients per coordinate (L+1)^2 */
+int memb_npts(const memb_plan* p);
+int memb_ncoef(const memb_plan* p);
+
+/* The FP64 law for ncap capsules (device pointers, capsule-major).  gs: shear modulus
+ * (gscell, param.f90:88); eb: bending modulus (ebend, param.f90:91; 0 = no bending).
+ * Writes the traction fx,fy,fz (load per unit area, forcing.f90:97-99), the filtered unit
+ * normal nx,ny,nz (forcing.f90:100-102) and the point area (forcing.f90:91,96). */
+int memb_traction_f64_dev(memb_plan* p, int64_t ncap, double gs, double eb, const double* x,
+                          const double* y, const double* z, double* fx, double* fy, double* fz,
+                          double* nx, double* ny, double* nz, double* area);
+
+/* Shape coefficients of ncap capsules (device): coef[c * 3 (L+1)^2 + ...], packed as above.
+ * The analysis of cal_3dcrd_spectral_coeff (sph1.f90:29-37). */
+int memb_coeffs_f64_dev(memb_plan* p, int64_t ncap, const double* x, const double* y,
+                        const double* z, double* coef);
+
+/* Geometry only (no law): the unit normal (x_theta x x_phi)/|.| and the point area on the
+ * force grid, for capsules whose load comes from the surrogate.  The area equals the law's;
+ * the normal is taken on the force grid instead of being filtered from the 2x grid. */
+int memb_geometry_f64_dev(memb_plan* p, int64_t ncap, const double* x, const double* y,
+                          const double* z, double* nx, double* ny, double* nz, double* area);
+
+/* ---- learned surrogate: MLP, BF16 MFMA with FP32 accumulation ----------------------------- */
+/* Layers: din -> w (SiLU) -> w (SiLU) -> ... (nhidden layers of width w) -> dout, linear
+ * output.  Weights are FP32 masters; the GEMMs read BF16 copies, refreshed whenever the
+ * masters change (set_layer, init, each Adam update).  Every GEMM dimension is padded
+ * internally to a multiple of 64.  The loss is taken on the scaled outputs y / out_scale. */
+typedef struct memb_mlp memb_mlp;
+int memb_mlp_create(memb_mlp** out, int din, int width, int nhidden, int dout, int max_batch,
+                    int device);
+void memb_mlp_destroy(memb_mlp* m);
+const char* memb_mlp_last_error(const memb_mlp* m);
+int memb_mlp_set_stream(memb_mlp* m, void* hip_stream);
+/* number of layers (nhidden + 1) and the shape of layer i: W_i is out x in, row-major */
+int memb_mlp_num_layers(const memb_mlp* m);
+int memb_mlp_layer_shape(const memb_mlp* m, int i, int* out_dim, int* in_dim);
+/* copy FP32 weights/biases of layer i in or out (host pointers) */
+int memb_mlp_set_layer(memb_mlp* m, int i, const float* W, const float* b);
+int memb_mlp_get_layer(memb_mlp* m, int i, float* W, float* b);
+/* input normalisation x' = (x - mean) * scale (din values each) and output scale: the network
+ * works on x' and predicts y / out_scale */
+int memb_mlp_set_norm(memb_mlp* m, const float* mean, const float* scale, float out_scale);
+/* seeded init: W ~ U(-a, a), a = sqrt(6 / (in + out)) (SiLU layers) ; b = 0 */
+int memb_mlp_init(memb_mlp* m, uint64_t seed);
+/* forward for `batch` rows: x[batch][din] FP32 (device) -> y[batch][dout] FP32 (device) */
+int memb_mlp_forward(memb_mlp* m, int batch, const float* x, float* y);
+/* one training step on a batch: forward, loss = mean over rows and outputs of
+ * (y - t)^2 * wt (wt[dout] FP32 per-output weight, NULL = 1), backward, Adam update
+ * (lr, beta1 0.9, beta2 0.999, eps 1e-8).  *loss (host) receives the batch loss. */
+int memb_mlp_train_step(memb_mlp* m, int batch, const float* x, const float* t,
+                        const float* wt, float lr, float* loss);
+/* gradients of the loss of the last train step (host copy, FP32), for checking */
+int memb_mlp_get_grad(memb_mlp* m, int i, float* dW, float* db);
+/* run the step without the Adam update (gradients only) */
+int memb_mlp_grad_only(memb_mlp* m, int batch, const float* x, const float* t, const float* wt,
+                       float* loss);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MEMB_H */

@@ -114,3 +114,64 @@ def load() -> ctypes.CDLL:
         f.argtypes = args
     _lib = lib
     return lib
+
+
+# ---- libmemb.so: the membrane law and its learned surrogate (include/memb.h) ----------------
+MEMB_PATH = os.path.join(HERE, "libmemb.so")
+MEMB_HEADER = os.path.join(ROOT, "include", "memb.h")
+_memb = None
+
+
+def declared_memb_symbols(header: str = MEMB_HEADER):
+    """Names of the functions include/memb.h declares."""
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(memb_[a-z0-9_]+)\s*\(", text)))
+
+
+def load_memb() -> ctypes.CDLL:
+    global _memb
+    if _memb is not None:
+        return _memb
+    if not os.path.exists(MEMB_PATH):
+        raise ImportError(
+            f"{MEMB_PATH} is missing: build it with `python -m suguru_amd.build` "
+            "(there is no fallback implementation)")
+    lib = ctypes.CDLL(MEMB_PATH)
+    P = ctypes.c_void_p
+    I64 = ctypes.c_int64
+    D = ctypes.c_double
+    F = ctypes.c_float
+    I = ctypes.c_int
+    sig = {
+        "memb_plan_create": (I, [ctypes.POINTER(P), I, I]),
+        "memb_plan_destroy": (None, [P]),
+        "memb_last_error": (ctypes.c_char_p, [P]),
+        "memb_set_stream": (I, [P, P]),
+        "memb_sync": (I, [P]),
+        "memb_npts": (I, [P]),
+        "memb_ncoef": (I, [P]),
+        "memb_traction_f64_dev": (I, [P, I64, D, D] + [P] * 10),
+        "memb_coeffs_f64_dev": (I, [P, I64, P, P, P, P]),
+        "memb_geometry_f64_dev": (I, [P, I64, P, P, P, P, P, P, P]),
+        "memb_mlp_create": (I, [ctypes.POINTER(P), I, I, I, I, I, I]),
+        "memb_mlp_destroy": (None, [P]),
+        "memb_mlp_last_error": (ctypes.c_char_p, [P]),
+        "memb_mlp_set_stream": (I, [P, P]),
+        "memb_mlp_num_layers": (I, [P]),
+        "memb_mlp_layer_shape": (I, [P, I, ctypes.POINTER(I), ctypes.POINTER(I)]),
+        "memb_mlp_set_layer": (I, [P, I, P, P]),
+        "memb_mlp_get_layer": (I, [P, I, P, P]),
+        "memb_mlp_set_norm": (I, [P, P, P, F]),
+        "memb_mlp_init": (I, [P, ctypes.c_uint64]),
+        "memb_mlp_forward": (I, [P, I, P, P]),
+        "memb_mlp_train_step": (I, [P, I, P, P, P, F, ctypes.POINTER(F)]),
+        "memb_mlp_get_grad": (I, [P, I, P, P]),
+        "memb_mlp_grad_only": (I, [P, I, P, P, P, ctypes.POINTER(F)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _memb = lib
+    return lib

@@ -17,12 +17,17 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libibx.so")
 SOURCES = [os.path.join(HERE, "csrc", "ibx.hip")]
 DEPS = SOURCES + [os.path.join(HERE, "csrc", "ibx_device.hpp"), os.path.join(ROOT, "include", "ibx.h")]
+# the capsule-membrane library: FP64 membrane law and the BF16-MFMA surrogate
+MEMB_LIB = os.path.join(HERE, "libmemb.so")
+MEMB_SOURCES = [os.path.join(HERE, "csrc", "membrane.hip"), os.path.join(HERE, "csrc", "mlp.hip")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("IBX_OFFLOAD_ARCH", "gfx950")
 
 
-def hipcc_cmd(out: str = LIB):
+def hipcc_cmd(out: str = LIB, sources=None, contract: bool = False):
+    """contract=False: no FMA contraction (the transfer library is bit-exact to the reference
+    loop order); the membrane library is checked to a tolerance and may contract."""
     return [
         HIPCC,
         f"--offload-arch={ARCH}",
@@ -30,23 +35,28 @@ def hipcc_cmd(out: str = LIB):
         "-std=c++17",
         "-fPIC",
         "-shared",
-        "-ffp-contract=off",
+        "-ffp-contract=fast" if contract else "-ffp-contract=off",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
         "-Wall",
         "-Wno-unused-function",
         "-Wno-unused-variable",
         "-o", out,
-        *SOURCES,
+        *(sources or SOURCES),
     ]
 
 
-def build(verbose: bool = True) -> str:
-    tmp = LIB + ".tmp"
-    cmd = hipcc_cmd(tmp)
+def _compile(cmd, out, verbose):
+    tmp = out + ".tmp"
+    cmd = [tmp if c == out else c for c in cmd]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
+
+
+def build(verbose: bool = True) -> str:
+    _compile(hipcc_cmd(LIB), LIB, verbose)
+    _compile(hipcc_cmd(MEMB_LIB, MEMB_SOURCES, contract=True), MEMB_LIB, verbose)
     return LIB
 
 

@@ -1,0 +1,894 @@
+// membrane.hip -- the reference's FP64 capsule membrane law on gfx950, one workgroup per
+// capsule (libmemb.so, C ABI in include/memb.h).
+//
+// Per capsule (forcing.f90:11-114 with the force grid equal to the harmonic grid):
+//   1. shape coefficients: longitude DFT per latitude, then Gauss-Legendre quadrature
+//      (the SPHEREPACK analysis of sph1.f90:29-79);
+//   2. on the 2x (dealias) grid, latitude ring by ring: x and all its derivatives up to 4th
+//      order (sph3.f90:9-182), then per node the geometry (normal, metrics, curvature,
+//      Christoffel symbols and their derivatives) and the load q1 a1 + q2 a2 + q3 n of the
+//      neo-Hookean tension and the bending moment (sph.f90:1523-1700);
+//   3. the dealias filter: analysis of the 2x-grid load and normal, synthesis of degrees
+//      <= L on the force grid (sph1.f90:723-951, 1378-1406), and the point area from the
+//      surface metric (sph3.f90:186-260; forcing.f90:91).
+// Everything is FP64.  Tables (Gauss nodes, Legendre functions and their theta-derivatives,
+// the unit-sphere reference state) are built once per degree by memb_plan_create.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/memb.h"
+
+namespace memb {
+
+constexpr int NQ = 15;        // derivatives of total order <= 4
+constexpr int RING = 3;       // fine latitude rings processed together
+constexpr int THREADS = 256;
+constexpr int NREF = 56;      // reference-state doubles per fine node
+constexpr int LMAX = 23;
+
+__host__ __device__ inline int qidx(int kt, int kp) {
+  const int s = kt + kp;
+  return s * (s + 1) / 2 + kp;
+}
+
+struct Tables {
+  int L, N, np2, nt, np;       // N = L+1 coarse latitudes; nt = 2N, np = 2 np2
+  const double* w2;            // [N] coarse Gauss weights
+  const double* w;             // [nt] fine Gauss weights
+  const double* s2;            // [N] sin(theta) coarse
+  const double* Pc;            // [N][N][N] (i, m, n) coarse Pbar
+  const double* Pcd;           // [N][N][N] coarse dPbar/dtheta
+  const double* Pfd;           // [nt][5][N][N] fine theta-derivatives 0..4
+  const double* Pfa;           // [nt][N][N] fine Pbar, degrees <= L
+  const double* cs2;           // [np2][N] cos(m phi_j) coarse
+  const double* sn2;
+  const double* csf;           // [np][N] fine
+  const double* snf;
+  const double* ref;           // [nt*np][NREF] unit-sphere state (mode 0 reads it)
+};
+
+// ---- per-node geometry and law (device versions of the oracle's formulas) ------------------
+struct Geom {
+  double a[2][3], n[3], J;
+  double g[2][2], gd[2][2][2];
+  double G[2][2], Gd[2][2][2], Gdd[2][2][2][2];
+  double b[2][2], bd[2][2][2], bdd[2][2][2][2];
+  double Gam[2][2][2], Gamd[2][2][2][2];
+};
+
+__device__ inline double dot3(const double* u, const double* v) {
+  return u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+}
+__device__ inline void cross3(const double* u, const double* v, double* r) {
+  r[0] = u[1] * v[2] - u[2] * v[1];
+  r[1] = u[2] * v[0] - u[0] * v[2];
+  r[2] = u[0] * v[1] - u[1] * v[0];
+}
+__device__ inline void mul2(const double X[2][2], const double Y[2][2], double R[2][2]) {
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) R[i][j] = X[i][0] * Y[0][j] + X[i][1] * Y[1][j];
+}
+
+// X[c][q]: derivatives of coordinate c; returns the node geometry
+__device__ void geometry(const double (*X)[NQ], Geom& o) {
+  double x1[2][3], x2[2][2][3], x3[2][2][2][3], x4[2][2][2][2][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      x1[a][k] = X[k][qidx(1 - a, a)];
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        x2[a][b][k] = X[k][qidx(2 - a - b, a + b)];
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          x3[a][b][c][k] = X[k][qidx(3 - a - b - c, a + b + c)];
+#pragma unroll
+          for (int d = 0; d < 2; ++d) x4[a][b][c][d][k] = X[k][qidx(4 - a - b - c - d, a + b + c + d)];
+        }
+      }
+    }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o.a[a][k] = x1[a][k];
+  double cv[3], nd[2][3];
+  cross3(x1[0], x1[1], cv);
+  o.J = sqrt(dot3(cv, cv));
+#pragma unroll
+  for (int k = 0; k < 3; ++k) o.n[k] = cv[k] / o.J;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    double t1[3], t2[3], cd[3];
+    cross3(x2[0][c], x1[1], t1);
+    cross3(x1[0], x2[1][c], t2);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) cd[k] = t1[k] + t2[k];
+    const double Jd = dot3(o.n, cd);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) nd[c][k] = (cd[k] - Jd * o.n[k]) / o.J;
+  }
+  double gdd[2][2][2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      o.g[a][b] = dot3(x1[a], x1[b]);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        o.gd[c][a][b] = dot3(x2[a][c], x1[b]) + dot3(x1[a], x2[b][c]);
+#pragma unroll
+        for (int d = 0; d < 2; ++d)
+          gdd[c][d][a][b] = dot3(x3[a][c][d], x1[b]) + dot3(x2[a][c], x2[b][d]) +
+                            dot3(x2[a][d], x2[b][c]) + dot3(x1[a], x3[b][c][d]);
+      }
+    }
+  const double det = o.g[0][0] * o.g[1][1] - o.g[0][1] * o.g[1][0];
+  o.G[0][0] = o.g[1][1] / det, o.G[1][1] = o.g[0][0] / det;
+  o.G[0][1] = -o.g[0][1] / det, o.G[1][0] = -o.g[1][0] / det;
+  double GdgG[2][2][2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    double t[2][2], u[2][2];
+    mul2(o.G, o.gd[c], t);
+    mul2(t, o.G, u);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) GdgG[c][a][b] = u[a][b], o.Gd[c][a][b] = -u[a][b];
+  }
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int d = 0; d < 2; ++d) {
+      double t1[2][2], t2[2][2], t3[2][2], t4[2][2], t5[2][2], gcd[2][2];
+      mul2(GdgG[d], o.gd[c], t1);
+      mul2(t1, o.G, t2);
+      mul2(GdgG[c], o.gd[d], t3);
+      mul2(t3, o.G, t4);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) gcd[a][b] = gdd[c][d][a][b];
+      mul2(o.G, gcd, t1);
+      mul2(t1, o.G, t5);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) o.Gdd[c][d][a][b] = t2[a][b] + t4[a][b] - t5[a][b];
+    }
+  double ac[2][3], acd[2][2][3];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      ac[a][k] = o.G[a][0] * x1[0][k] + o.G[a][1] * x1[1][k];
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        acd[c][a][k] = o.Gd[c][a][0] * x1[0][k] + o.Gd[c][a][1] * x1[1][k] +
+                       o.G[a][0] * x2[0][c][k] + o.G[a][1] * x2[1][c][k];
+    }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      o.b[a][b] = dot3(x2[a][b], o.n);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) o.bd[c][a][b] = dot3(x3[a][b][c], o.n) + dot3(x2[a][b], nd[c]);
+    }
+  double ndd[2][2][3];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int d = 0; d < 2; ++d)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int l = 0; l < 2; ++l) s += o.bd[d][c][l] * ac[l][k] + o.b[c][l] * acd[d][l][k];
+        ndd[c][d][k] = -s;
+      }
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int d = 0; d < 2; ++d)
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+          o.bdd[c][d][a][b] = dot3(x4[a][b][c][d], o.n) + dot3(x3[a][b][c], nd[d]) +
+                              dot3(x3[a][b][d], nd[c]) + dot3(x2[a][b], ndd[c][d]);
+#pragma unroll
+  for (int b = 0; b < 2; ++b)
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        o.Gam[b][a][c] = dot3(x2[a][c], ac[b]);
+#pragma unroll
+        for (int l = 0; l < 2; ++l)
+          o.Gamd[l][b][a][c] = dot3(x3[a][c][l], ac[b]) + dot3(x2[a][c], acd[l][b]);
+      }
+}
+
+// load q1 a1 + q2 a2 + q3 n at one node (sph.f90:1615-1700; membmodel.c:60-79)
+__device__ void law(const Geom& o, const double* __restrict__ R, double gs, double eb,
+                    double f[3]) {
+  double A[2][2], Ad[2][2][2], Add[2][2][2][2], B[2][2], Bd[2][2][2], Bdd[2][2][2][2];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) (&A[0][0])[q] = R[q], (&B[0][0])[q] = R[28 + q];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) (&Ad[0][0][0])[q] = R[4 + q], (&Bd[0][0][0])[q] = R[32 + q];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) (&Add[0][0][0][0])[q] = R[12 + q], (&Bdd[0][0][0][0])[q] = R[40 + q];
+  const double detA = A[0][0] * A[1][1] - A[0][1] * A[0][1];
+  const double deta = o.g[0][0] * o.g[1][1] - o.g[0][1] * o.g[0][1];
+  const double I2 = detA * deta - 1.0;
+  const double Js = sqrt(1.0 + I2);
+  double I2d[2], Jsd[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const double dA = Ad[c][0][0] * A[1][1] + A[0][0] * Ad[c][1][1] - 2.0 * A[0][1] * Ad[c][0][1];
+    const double da = o.gd[c][0][0] * o.g[1][1] + o.g[0][0] * o.gd[c][1][1] -
+                      2.0 * o.g[0][1] * o.gd[c][0][1];
+    I2d[c] = dA * deta + da * detA;
+    Jsd[c] = 0.5 / Js * I2d[c];
+  }
+  const double w1 = 0.5 * gs;
+  const double w2 = -0.5 * gs / ((I2 + 1.0) * (I2 + 1.0));
+  double w2d[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) w2d[c] = gs / ((I2 + 1.0) * (I2 + 1.0) * (I2 + 1.0)) * I2d[c];
+  double T[2][2], Td[2][2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      T[a][b] = 2.0 / Js * w1 * A[a][b] + 2.0 * Js * w2 * o.G[a][b];
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        Td[c][a][b] = -2.0 / (Js * Js) * Jsd[c] * w1 * A[a][b] + 2.0 / Js * w1 * Ad[c][a][b] +
+                      2.0 * Jsd[c] * w2 * o.G[a][b] + 2.0 * Js * w2d[c] * o.G[a][b] +
+                      2.0 * Js * w2 * o.Gd[c][a][b];
+    }
+  double q[2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    double s = Td[0][0][b] + Td[1][1][b];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int l = 0; l < 2; ++l) s += o.Gam[a][a][l] * T[l][b] + o.Gam[b][a][l] * T[a][l];
+    q[b] = s;
+  }
+  double q3 = 0.0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) q3 += T[a][b] * o.b[a][b];
+  if (eb != 0.0) {
+    double K[2][2], Kd[2][2][2], Kdd[2][2][2][2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        double s = 0.0;
+#pragma unroll
+        for (int cp = 0; cp < 2; ++cp) s += A[cp][a] * B[c][cp] - o.G[cp][a] * o.b[c][cp];
+        K[c][a] = s;
+#pragma unroll
+        for (int l = 0; l < 2; ++l) {
+          double t = 0.0;
+#pragma unroll
+          for (int cp = 0; cp < 2; ++cp)
+            t += Ad[l][cp][a] * B[c][cp] + A[cp][a] * Bd[l][c][cp] - o.Gd[l][cp][a] * o.b[c][cp] -
+                 o.G[cp][a] * o.bd[l][c][cp];
+          Kd[l][c][a] = t;
+#pragma unroll
+          for (int k = 0; k < 2; ++k) {
+            double u = 0.0;
+#pragma unroll
+            for (int cp = 0; cp < 2; ++cp)
+              u += Add[l][k][cp][a] * B[c][cp] + Ad[k][cp][a] * Bd[l][c][cp] +
+                   Ad[l][cp][a] * Bd[k][c][cp] + A[cp][a] * Bdd[l][k][c][cp] -
+                   o.Gdd[l][k][cp][a] * o.b[c][cp] - o.Gd[k][cp][a] * o.bd[l][c][cp] -
+                   o.Gd[l][cp][a] * o.bd[k][c][cp] - o.G[cp][a] * o.bdd[l][k][c][cp];
+            Kdd[l][k][c][a] = u;
+          }
+        }
+      }
+    double M[2][2], Md[2][2][2], dds = 0.0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        double m = 0.0, md0 = 0.0, md1 = 0.0;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          m += o.G[b][c] * K[c][a];
+          md0 += o.Gd[0][b][c] * K[c][a] + o.G[b][c] * Kd[0][c][a];
+          md1 += o.Gd[1][b][c] * K[c][a] + o.G[b][c] * Kd[1][c][a];
+          dds += o.Gdd[b][a][b][c] * K[c][a] + o.Gd[a][b][c] * Kd[b][c][a] +
+                 o.Gd[b][b][c] * Kd[a][c][a] + o.G[b][c] * Kdd[b][a][c][a];
+        }
+        M[a][b] = m, Md[0][a][b] = md0, Md[1][a][b] = md1;
+      }
+    M[1][0] = M[0][1];
+    Md[0][1][0] = Md[0][0][1], Md[1][1][0] = Md[1][0][1];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        M[a][b] *= eb;
+        Md[0][a][b] *= eb, Md[1][a][b] *= eb;
+      }
+    dds *= eb;
+    double Q[2], Qc = dds;
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      double s = 0.0;
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        s += Md[a][a][b];
+#pragma unroll
+        for (int l = 0; l < 2; ++l) {
+          s += o.Gam[a][a][l] * M[l][b] + o.Gam[b][a][l] * M[a][l];
+          Qc += o.Gamd[b][a][a][l] * M[l][b] + o.Gam[a][a][l] * Md[b][l][b] +
+                o.Gamd[b][b][a][l] * M[a][l] + o.Gam[b][a][l] * Md[b][a][l];
+        }
+      }
+      Q[b] = s;
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int l = 0; l < 2; ++l) Qc += Q[l] * o.Gam[b][b][l];
+    double bm[2][2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int a = 0; a < 2; ++a) bm[b][a] = o.G[0][a] * o.b[b][0] + o.G[1][a] * o.b[b][1];
+    q[0] -= bm[0][0] * Q[0] + bm[1][0] * Q[1];
+    q[1] -= bm[0][1] * Q[0] + bm[1][1] * Q[1];
+    q3 += Qc;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) f[k] = q[0] * o.a[0][k] + q[1] * o.a[1][k] + q3 * o.n[k];
+}
+
+// ---- the per-capsule kernel ---------------------------------------------------------------
+// MODE 0: the law; 1: the reference state of one capsule (the unit sphere) into `ref`;
+// 2: shape coefficients only; 3: geometry only (unit normal and point area on the force
+// grid, no load).  Persistent: workgroup b handles capsules b, b + gridDim.x, ...
+// LDS: coarse coefficients [3][N][N] a and b, then a work region reused by the phases.
+// scratch: per workgroup, longitude DFT of the 2x-grid outputs [nt][6][N][2].
+template <int MODE>
+__global__ void __launch_bounds__(THREADS)
+    k_mem(Tables T, long long ncap, double gs, double eb, const double* __restrict__ x,
+          const double* __restrict__ y, const double* __restrict__ z, double* __restrict__ o0,
+          double* __restrict__ o1, double* __restrict__ o2, double* __restrict__ o3,
+          double* __restrict__ o4, double* __restrict__ o5, double* __restrict__ o6,
+          double* __restrict__ scratch) {
+  extern __shared__ double lds[];
+  const int tid = threadIdx.x;
+  const int N = T.N, NN = N * N, np2 = T.np2, nt = T.nt, np = T.np;
+  const int nc = N * np2;
+  double* ca = lds;            // [3][N][N]
+  double* cb = ca + 3 * NN;    // [3][N][N]
+  double* U = cb + 3 * NN;     // work region
+  double* scr = scratch + (size_t)blockIdx.x * nt * 6 * N * 2;  // >= 6 nc doubles too
+  for (long long cap = blockIdx.x; cap < ncap; cap += gridDim.x) {
+    const double* xyz[3] = {x + cap * nc, y + cap * nc, z + cap * nc};
+    // 1. analysis: longitude DFT per (c, i, m) into U, then quadrature per (c, m, n)
+    for (int t = tid; t < 3 * NN; t += THREADS) {
+      const int c = t / NN, i = (t / N) % N, m = t % N;
+      const double* f = xyz[c] + i * np2;
+      double cs = 0.0, sn = 0.0;
+      for (int j = 0; j < np2; ++j) {
+        const double v = f[j];
+        cs += v * T.cs2[j * N + m];
+        sn += v * T.sn2[j * N + m];
+      }
+      U[2 * t] = cs, U[2 * t + 1] = sn;
+    }
+    __syncthreads();
+    for (int t = tid; t < 3 * NN; t += THREADS) {
+      const int c = t / NN, m = (t / N) % N, n = t % N;
+      double sa = 0.0, sb = 0.0;
+      if (n >= m)
+        for (int i = 0; i < N; ++i) {
+          const double p = T.w2[i] * T.Pc[(i * N + m) * N + n];
+          const int u = (c * N + i) * N + m;
+          sa += p * U[2 * u];
+          sb += p * U[2 * u + 1];
+        }
+      ca[(c * N + m) * N + n] = 2.0 / np2 * sa;
+      cb[(c * N + m) * N + n] = -2.0 / np2 * sb;
+    }
+    __syncthreads();
+    if (MODE == 2) {
+      // packed: a(m, n>=m) then b(m>=1, n>=m), per coordinate
+      for (int t = tid; t < 3 * NN; t += THREADS) {
+        const int c = t / NN, m = (t / N) % N, n = t % N;
+        if (n < m) continue;
+        const int pa = m * N - m * (m - 1) / 2 + (n - m);
+        o0[cap * 3 * NN + c * NN + pa] = ca[(c * N + m) * N + n];
+        if (m >= 1) {
+          const int pb = (N * (N + 1)) / 2 + (m - 1) * N - (m - 1) * (m - 2) / 2 - (m - 1) + (n - m);
+          o0[cap * 3 * NN + c * NN + pb] = cb[(c * N + m) * N + n];
+        }
+      }
+      __syncthreads();
+      continue;
+    }
+    // 2. the 2x grid, RING latitudes at a time (not for MODE 3: geometry only)
+    double* AB = U;                          // [RING][3][5][N][2]
+    double* F = AB + RING * 3 * 5 * N * 2;   // [RING][np][6]
+    for (int i0 = 0; i0 < (MODE == 3 ? 0 : nt); i0 += RING) {
+      const int nr = (nt - i0) < RING ? (nt - i0) : RING;
+      for (int t = tid; t < nr * 3 * 5 * N; t += THREADS) {
+        const int r = t / (15 * N), c = (t / (5 * N)) % 3, k = (t / N) % 5, m = t % N;
+        const double* Pd = T.Pfd + (((size_t)(i0 + r) * 5 + k) * N + m) * N;
+        const double h = m == 0 ? 0.5 : 1.0;
+        double sa = 0.0, sb = 0.0;
+        for (int n = m; n < N; ++n) {
+          sa += Pd[n] * ca[(c * N + m) * N + n];
+          sb += Pd[n] * cb[(c * N + m) * N + n];
+        }
+        AB[2 * t] = h * sa, AB[2 * t + 1] = h * sb;
+      }
+      __syncthreads();
+      for (int t = tid; t < nr * np; t += THREADS) {
+        const int r = t / np, j = t % np;
+        double X[3][NQ];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) X[c][q] = 0.0;
+        for (int m = 0; m < N; ++m) {
+          const double cm = T.csf[j * N + m], sm = T.snf[j * N + m];
+          const double m1 = (double)m, m2 = m1 * m1, m3 = m2 * m1, m4 = m2 * m2;
+#pragma unroll
+          for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int kt = 0; kt <= 4; ++kt) {
+              const int ab = ((r * 3 + c) * 5 + kt) * N + m;
+              const double A = AB[2 * ab], B = AB[2 * ab + 1];
+              const double h0 = A * cm - B * sm, h1 = -A * sm - B * cm;
+              X[c][qidx(kt, 0)] += h0;
+              if (kt <= 3) X[c][qidx(kt, 1)] += m1 * h1;
+              if (kt <= 2) X[c][qidx(kt, 2)] -= m2 * h0;
+              if (kt <= 1) X[c][qidx(kt, 3)] -= m3 * h1;
+              if (kt == 0) X[c][qidx(0, 4)] += m4 * h0;
+            }
+        }
+        Geom o;
+        geometry(X, o);
+        const size_t node = (size_t)(i0 + r) * np + j;
+        if (MODE == 1) {
+          double* R = o0 + node * NREF;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) R[q] = (&o.G[0][0])[q], R[28 + q] = (&o.b[0][0])[q];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) R[4 + q] = (&o.Gd[0][0][0])[q], R[32 + q] = (&o.bd[0][0][0])[q];
+#pragma unroll
+          for (int q = 0; q < 16; ++q)
+            R[12 + q] = (&o.Gdd[0][0][0][0])[q], R[40 + q] = (&o.bdd[0][0][0][0])[q];
+        } else {
+          double f[3];
+          law(o, T.ref + node * NREF, gs, eb, f);
+          double* Fo = F + (size_t)t * 6;
+          Fo[0] = f[0], Fo[1] = f[1], Fo[2] = f[2];
+          Fo[3] = o.n[0], Fo[4] = o.n[1], Fo[5] = o.n[2];
+        }
+      }
+      __syncthreads();
+      if (MODE == 0) {
+        // longitude DFT of the ring's six outputs, degrees <= L only
+        for (int t = tid; t < nr * 6 * N; t += THREADS) {
+          const int r = t / (6 * N), fi = (t / N) % 6, m = t % N;
+          double cs = 0.0, sn = 0.0;
+          for (int j = 0; j < np; ++j) {
+            const double v = F[((size_t)r * np + j) * 6 + fi];
+            cs += v * T.csf[j * N + m];
+            sn += v * T.snf[j * N + m];
+          }
+          double* s = scr + (((size_t)(i0 + r) * 6 + fi) * N + m) * 2;
+          s[0] = cs, s[1] = sn;
+        }
+        __syncthreads();
+      }
+    }
+    if (MODE == 1) continue;
+    // 3. filter: quadrature of the 2x-grid outputs (degrees <= L), synthesis on the force
+    //    grid, field by field; the area from the coarse shape (sph3.f90:186-260)
+    double* fc = U;              // [N][N][2] coefficients of the current field
+    double* tmp = U + 2 * NN;    // [N][N][2] per (i, m) partial sums
+    double* xt = scr;            // [nc][3] x_theta on the force grid (scratch reused)
+    double* xp = scr + (size_t)nc * 3;  // [nc][3] x_phi
+    for (int fi = MODE == 3 ? 6 : 0; fi < 12; ++fi) {
+      if (fi < 6) {
+        for (int t = tid; t < NN; t += THREADS) {
+          const int m = t / N, n = t % N;
+          double sa = 0.0, sb = 0.0;
+          if (n >= m)
+            for (int i = 0; i < nt; ++i) {
+              const double p = T.w[i] * T.Pfa[((size_t)i * N + m) * N + n];
+              const double* s = scr + (((size_t)i * 6 + fi) * N + m) * 2;
+              sa += p * s[0];
+              sb += p * s[1];
+            }
+          fc[2 * t] = 2.0 / np * sa, fc[2 * t + 1] = -2.0 / np * sb;
+        }
+        __syncthreads();
+        for (int t = tid; t < NN; t += THREADS) {
+          const int i = t / N, m = t % N;
+          const double h = m == 0 ? 0.5 : 1.0;
+          double sa = 0.0, sb = 0.0;
+          for (int n = m; n < N; ++n) {
+            const double p = T.Pc[(i * N + m) * N + n];
+            sa += p * fc[2 * (m * N + n)];
+            sb += p * fc[2 * (m * N + n) + 1];
+          }
+          tmp[2 * t] = h * sa, tmp[2 * t + 1] = h * sb;
+        }
+      } else {
+        // fi 6..8: x_theta, fi 9..11: x_phi of coordinate c, from the coarse shape
+        const bool dth = fi < 9;
+        const int c = dth ? fi - 6 : fi - 9;
+        for (int t = tid; t < NN; t += THREADS) {
+          const int i = t / N, m = t % N;
+          const double h = m == 0 ? 0.5 : 1.0;
+          const double* P = (dth ? T.Pcd : T.Pc) + (i * N + m) * N;
+          double sa = 0.0, sb = 0.0;
+          for (int n = m; n < N; ++n) {
+            sa += P[n] * ca[(c * N + m) * N + n];
+            sb += P[n] * cb[(c * N + m) * N + n];
+          }
+          tmp[2 * t] = h * sa, tmp[2 * t + 1] = h * sb;
+        }
+      }
+      __syncthreads();
+      // the 2x-grid longitude DFT partials are dead once fi >= 6: x_theta / x_phi reuse them
+      double* dst = fi == 0 ? o0 : fi == 1 ? o1 : fi == 2 ? o2 : fi == 3 ? o3 : fi == 4 ? o4 : o5;
+      for (int node = tid; node < nc; node += THREADS) {
+        const int i = node / np2, j = node % np2;
+        double v = 0.0;
+        for (int m = 0; m < N; ++m) {
+          const double A = tmp[2 * (i * N + m)], B = tmp[2 * (i * N + m) + 1];
+          const double cm = T.cs2[j * N + m], sm = T.sn2[j * N + m];
+          if (fi < 9)
+            v += A * cm - B * sm;
+          else
+            v += (double)m * (-A * sm - B * cm);
+        }
+        if (fi < 6)
+          dst[(size_t)cap * nc + node] = v;
+        else if (fi < 9)
+          xt[(size_t)node * 3 + (fi - 6)] = v;
+        else
+          xp[(size_t)node * 3 + (fi - 9)] = v;
+      }
+      __syncthreads();
+    }
+    // unit normal (cal_unitO_filter, sph1.f90:1378-1406) and point area
+    // surfmet * w_i / sin(theta_i) * dphi (forcing.f90:91; sph.f90:131)
+    for (int node = tid; node < nc; node += THREADS) {
+      const size_t o = (size_t)cap * nc + node;
+      double cr[3];
+      cross3(xt + (size_t)node * 3, xp + (size_t)node * 3, cr);
+      const double J = sqrt(dot3(cr, cr));
+      if (MODE == 0) {
+        const double a = o3[o], b = o4[o], c = o5[o];
+        const double r = sqrt(a * a + b * b + c * c);
+        o3[o] = a / r, o4[o] = b / r, o5[o] = c / r;
+      } else {
+        // MODE 3: the unit normal of the force grid itself, (x_t x x_p) / |x_t x x_p|
+        o3[o] = cr[0] / J, o4[o] = cr[1] / J, o5[o] = cr[2] / J;
+      }
+      const int i = node / np2;
+      o6[o] = J * T.w2[i] / T.s2[i] * (2.0 * M_PI / np2);
+    }
+    __syncthreads();
+  }
+}
+
+// ---- host tables ------------------------------------------------------------------------
+void gauss(int n, double* theta, double* w) {
+  for (int i = 0; i < n; ++i) {
+    double x = std::cos(M_PI * (i + 0.75) / (n + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = x;
+      for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+        p0 = p1, p1 = p2;
+      }
+      const double pn = n == 1 ? x : p1, pm = n == 1 ? 1.0 : p0;
+      dp = n * (x * pn - pm) / (x * x - 1.0);
+      const double dx = pn / dp;
+      x -= dx;
+      if (std::fabs(dx) < 1e-16) {
+        p0 = 1.0, p1 = x;
+        for (int k = 2; k <= n; ++k) {
+          const double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+          p0 = p1, p1 = p2;
+        }
+        const double qn = n == 1 ? x : p1, qm = n == 1 ? 1.0 : p0;
+        dp = n * (x * qn - qm) / (x * x - 1.0);
+        break;
+      }
+    }
+    theta[i] = std::acos(x);
+    w[i] = 2.0 / ((1.0 - x * x) * dp * dp);
+  }
+}
+
+// Pbar_n^m(cos t), m <= n < nmax: P[m * nmax + n]
+void legendre(int nmax, double t, double* P) {
+  const double c = std::cos(t), s = std::sin(t);
+  std::fill(P, P + nmax * nmax, 0.0);
+  double pmm = std::sqrt(0.5);
+  for (int m = 0; m < nmax; ++m) {
+    if (m > 0) pmm *= std::sqrt((2.0 * m + 1.0) / (2.0 * m)) * s;
+    P[m * nmax + m] = pmm;
+    if (m + 1 < nmax) P[m * nmax + m + 1] = std::sqrt(2.0 * m + 3.0) * c * pmm;
+    for (int n = m + 2; n < nmax; ++n) {
+      const double a = std::sqrt((4.0 * n * n - 1.0) / ((double)n * n - (double)m * m));
+      const double a1 =
+          std::sqrt((4.0 * (n - 1.0) * (n - 1.0) - 1.0) / ((n - 1.0) * (n - 1.0) - (double)m * m));
+      P[m * nmax + n] = a * (c * P[m * nmax + n - 1] - P[m * nmax + n - 2] / a1);
+    }
+  }
+}
+
+// theta-derivatives 0..4 by the ladder d/dt Pbar^m = (c-_m Pbar^{m-1} - c+_m Pbar^{m+1}) / 2
+void legendre_derivs(int nmax, double t, double* D /* [5][nmax][nmax] */) {
+  std::vector<double> P(nmax * nmax);
+  legendre(nmax, t, P.data());
+  std::fill(D, D + 5 * nmax * nmax, 0.0);
+  std::vector<double> v(5 * (2 * nmax + 3));
+  for (int n = 0; n < nmax; ++n) {
+    const int off = n + 1, W = 2 * nmax + 3;
+    std::fill(v.begin(), v.end(), 0.0);
+    for (int m = -n; m <= n; ++m) {
+      const int am = m < 0 ? -m : m;
+      v[m + off] = ((m < 0 && (am & 1)) ? -1.0 : 1.0) * P[am * nmax + n];
+    }
+    for (int k = 1; k < 5; ++k)
+      for (int m = -n; m <= n; ++m) {
+        const double cm = std::sqrt((double)(n + m) * (double)(n - m + 1));
+        const double cp = std::sqrt((double)(n + m + 1) * (double)(n - m));
+        v[k * W + m + off] = 0.5 * (cm * v[(k - 1) * W + m - 1 + off] - cp * v[(k - 1) * W + m + 1 + off]);
+      }
+    for (int k = 0; k < 5; ++k)
+      for (int m = 0; m <= n; ++m) D[(k * nmax + m) * nmax + n] = v[k * W + m + off];
+  }
+}
+
+}  // namespace memb
+
+using namespace memb;
+
+struct memb_plan {
+  int device = 0, L = 0;
+  hipStream_t own = nullptr, stream = nullptr;
+  std::string err;
+  Tables T{};
+  std::vector<void*> bufs;  // device allocations of the tables
+  double* ref = nullptr;
+  double* scratch = nullptr;
+  size_t scratch_wg = 0;
+  int num_wg = 0;
+  size_t lds = 0;
+};
+
+namespace {
+int fail(memb_plan* p, int code, const std::string& m) {
+  if (p) p->err = m;
+  return code;
+}
+#define MCHK(p, call)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail((p), MEMB_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+int upload(memb_plan* p, const std::vector<double>& h, const double** dst) {
+  void* d = nullptr;
+  MCHK(p, hipMalloc(&d, h.size() * sizeof(double)));
+  p->bufs.push_back(d);
+  MCHK(p, hipMemcpy(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  *dst = (const double*)d;
+  return MEMB_OK;
+}
+
+size_t lds_bytes(const Tables& T) {
+  const int N = T.N, NN = N * N;
+  size_t work = std::max<size_t>(3 * NN * 2, (size_t)RING * 3 * 5 * N * 2 + (size_t)RING * T.np * 6);
+  work = std::max<size_t>(work, 4 * NN);
+  return (6 * NN + work) * sizeof(double);
+}
+
+template <int MODE>
+int launch(memb_plan* p, long long ncap, double gs, double eb, const double* x, const double* y,
+           const double* z, double* o0, double* o1, double* o2, double* o3, double* o4,
+           double* o5, double* o6) {
+  const int nwg = (int)std::min<long long>(ncap, p->num_wg);
+  if (nwg <= 0) return MEMB_OK;
+  k_mem<MODE><<<nwg, THREADS, p->lds, p->stream>>>(p->T, ncap, gs, eb, x, y, z, o0, o1, o2, o3,
+                                                   o4, o5, o6, p->scratch);
+  MCHK(p, hipGetLastError());
+  return MEMB_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int memb_plan_create(memb_plan** out, int L, int device) {
+  if (!out || L < 2 || L > LMAX) return MEMB_E_ARG;
+  *out = nullptr;
+  memb_plan* p = new memb_plan();
+  p->device = device, p->L = L;
+  int rc;
+  auto bail = [&](int code) {
+    *out = p;  // keep the message readable through memb_last_error
+    return code;
+  };
+  if (hipSetDevice(device) != hipSuccess) return bail(fail(p, MEMB_E_HIP, "hipSetDevice failed"));
+  if (hipStreamCreate(&p->own) != hipSuccess)
+    return bail(fail(p, MEMB_E_HIP, "hipStreamCreate failed"));
+  p->stream = p->own;
+  Tables& T = p->T;
+  T.L = L, T.N = L + 1, T.np2 = 2 * (L + 1), T.nt = 2 * T.N, T.np = 2 * T.np2;
+  const int N = T.N, nt = T.nt, np = T.np, np2 = T.np2;
+  std::vector<double> th2(N), w2(N), th(nt), w(nt), s2(N);
+  gauss(N, th2.data(), w2.data());
+  gauss(nt, th.data(), w.data());
+  for (int i = 0; i < N; ++i) s2[i] = std::sin(th2[i]);
+  std::vector<double> Pc((size_t)N * N * N), Pcd((size_t)N * N * N), Pfd((size_t)nt * 5 * N * N),
+      Pfa((size_t)nt * N * N), D(5 * N * N);
+  for (int i = 0; i < N; ++i) {
+    legendre_derivs(N, th2[i], D.data());
+    std::copy(D.begin(), D.begin() + N * N, Pc.begin() + (size_t)i * N * N);
+    std::copy(D.begin() + N * N, D.begin() + 2 * N * N, Pcd.begin() + (size_t)i * N * N);
+  }
+  for (int i = 0; i < nt; ++i) {
+    legendre_derivs(N, th[i], D.data());
+    std::copy(D.begin(), D.end(), Pfd.begin() + (size_t)i * 5 * N * N);
+    std::copy(D.begin(), D.begin() + N * N, Pfa.begin() + (size_t)i * N * N);
+  }
+  std::vector<double> cs2((size_t)np2 * N), sn2((size_t)np2 * N), csf((size_t)np * N),
+      snf((size_t)np * N);
+  for (int j = 0; j < np2; ++j)
+    for (int m = 0; m < N; ++m) {
+      const double ph = 2.0 * M_PI * j / np2;
+      cs2[j * N + m] = std::cos(m * ph), sn2[j * N + m] = std::sin(m * ph);
+    }
+  for (int j = 0; j < np; ++j)
+    for (int m = 0; m < N; ++m) {
+      const double ph = 2.0 * M_PI * j / np;
+      csf[j * N + m] = std::cos(m * ph), snf[j * N + m] = std::sin(m * ph);
+    }
+  if ((rc = upload(p, w2, &T.w2)) || (rc = upload(p, w, &T.w)) || (rc = upload(p, s2, &T.s2)) ||
+      (rc = upload(p, Pc, &T.Pc)) || (rc = upload(p, Pcd, &T.Pcd)) ||
+      (rc = upload(p, Pfd, &T.Pfd)) || (rc = upload(p, Pfa, &T.Pfa)) ||
+      (rc = upload(p, cs2, &T.cs2)) || (rc = upload(p, sn2, &T.sn2)) ||
+      (rc = upload(p, csf, &T.csf)) || (rc = upload(p, snf, &T.snf)))
+    return bail(rc);
+  p->lds = lds_bytes(T);
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  p->num_wg = 2 * cus;
+  // per workgroup: longitude DFT of the 2x-grid outputs, and the force-grid basis vectors
+  p->scratch_wg = std::max<size_t>((size_t)nt * 6 * N * 2, (size_t)N * np2 * 6);
+  void* s = nullptr;
+  if (hipMalloc(&s, p->scratch_wg * p->num_wg * sizeof(double)) != hipSuccess)
+    return bail(fail(p, MEMB_E_HIP, "hipMalloc scratch failed"));
+  p->bufs.push_back(s);
+  p->scratch = (double*)s;
+  // the reference state: the unit sphere through the same kernel (sph.f90:56-70, 152-161)
+  void* r = nullptr;
+  if (hipMalloc(&r, (size_t)nt * np * NREF * sizeof(double)) != hipSuccess)
+    return bail(fail(p, MEMB_E_HIP, "hipMalloc reference state failed"));
+  p->bufs.push_back(r);
+  p->ref = (double*)r;
+  std::vector<double> sx((size_t)N * np2 * 3);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < np2; ++j) {
+      const double ph = 2.0 * M_PI * j / np2;
+      sx[i * np2 + j] = std::sin(th2[i]) * std::sin(ph);
+      sx[N * np2 + i * np2 + j] = std::sin(th2[i]) * std::cos(ph);
+      sx[2 * N * np2 + i * np2 + j] = std::cos(th2[i]);
+    }
+  const double* dsx = nullptr;
+  if ((rc = upload(p, sx, &dsx))) return bail(rc);
+  T.ref = nullptr;
+  // scratch stride is per workgroup; one capsule uses workgroup 0's region
+  if (hipFuncSetAttribute((const void*)k_mem<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)p->lds) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_mem<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)p->lds) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_mem<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)p->lds) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_mem<3>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)p->lds) != hipSuccess)
+    return bail(fail(p, MEMB_E_HIP, "hipFuncSetAttribute failed"));
+  if ((rc = launch<1>(p, 1, 0.0, 0.0, dsx, dsx + N * np2, dsx + 2 * N * np2, p->ref, nullptr,
+                      nullptr, nullptr, nullptr, nullptr, nullptr)))
+    return bail(rc);
+  if (hipStreamSynchronize(p->stream) != hipSuccess)
+    return bail(fail(p, MEMB_E_HIP, "reference state failed"));
+  T.ref = p->ref;
+  *out = p;
+  return MEMB_OK;
+}
+
+void memb_plan_destroy(memb_plan* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->device);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  for (void* b : p->bufs) (void)hipFree(b);
+  if (p->own) (void)hipStreamDestroy(p->own);
+  delete p;
+}
+
+const char* memb_last_error(const memb_plan* p) { return p ? p->err.c_str() : "null plan"; }
+
+int memb_set_stream(memb_plan* p, void* s) {
+  if (!p) return MEMB_E_ARG;
+  p->stream = s ? (hipStream_t)s : p->own;
+  return MEMB_OK;
+}
+
+int memb_sync(memb_plan* p) {
+  if (!p) return MEMB_E_ARG;
+  MCHK(p, hipSetDevice(p->device));
+  MCHK(p, hipStreamSynchronize(p->stream));
+  return MEMB_OK;
+}
+
+int memb_npts(const memb_plan* p) { return p ? p->T.N * p->T.np2 : 0; }
+int memb_ncoef(const memb_plan* p) { return p ? p->T.N * p->T.N : 0; }
+
+int memb_traction_f64_dev(memb_plan* p, int64_t ncap, double gs, double eb, const double* x,
+                          const double* y, const double* z, double* fx, double* fy, double* fz,
+                          double* nx, double* ny, double* nz, double* area) {
+  if (!p) return MEMB_E_ARG;
+  if (!p->T.ref) return fail(p, MEMB_E_STATE, "plan not initialised");
+  if (ncap < 0) return fail(p, MEMB_E_ARG, "ncap < 0");
+  if (ncap > 0 && !(x && y && z && fx && fy && fz && nx && ny && nz && area))
+    return fail(p, MEMB_E_ARG, "null pointer");
+  MCHK(p, hipSetDevice(p->device));
+  return launch<0>(p, ncap, gs, eb, x, y, z, fx, fy, fz, nx, ny, nz, area);
+}
+
+int memb_coeffs_f64_dev(memb_plan* p, int64_t ncap, const double* x, const double* y,
+                        const double* z, double* coef) {
+  if (!p) return MEMB_E_ARG;
+  if (ncap < 0) return fail(p, MEMB_E_ARG, "ncap < 0");
+  if (ncap > 0 && !(x && y && z && coef)) return fail(p, MEMB_E_ARG, "null pointer");
+  MCHK(p, hipSetDevice(p->device));
+  return launch<2>(p, ncap, 0.0, 0.0, x, y, z, coef, nullptr, nullptr, nullptr, nullptr, nullptr,
+                   nullptr);
+}
+
+int memb_geometry_f64_dev(memb_plan* p, int64_t ncap, const double* x, const double* y,
+                          const double* z, double* nx, double* ny, double* nz, double* area) {
+  if (!p) return MEMB_E_ARG;
+  if (ncap < 0) return fail(p, MEMB_E_ARG, "ncap < 0");
+  if (ncap > 0 && !(x && y && z && nx && ny && nz && area))
+    return fail(p, MEMB_E_ARG, "null pointer");
+  MCHK(p, hipSetDevice(p->device));
+  return launch<3>(p, ncap, 0.0, 0.0, x, y, z, nullptr, nullptr, nullptr, nx, ny, nz, area);
+}
+
+}  // extern "C"

@@ -175,15 +175,19 @@ int ibx_batch_bind(ibx_batch* b, const ibx_batch_bufs* bufs);
  * Writes the chosen configuration id into *choice (may be NULL). */
 int ibx_batch_autotune(ibx_batch* b, const char* cache_path, int* choice);
 int ibx_batch_set_config(ibx_batch* b, int choice);
+/* on != 0: run the interpolation on a second stream of the context, concurrently with the
+ * spread (they share only read-only inputs); the captured graph then has two branches. */
+int ibx_batch_set_overlap(ibx_batch* b, int on);
 int ibx_batch_num_configs(void);
 const char* ibx_batch_config_name(int choice);
 /* Capture interp + spread on the context stream into a graph (use_graph = 1) or run eagerly
  * (use_graph = 0) on every ibx_batch_run. */
 int ibx_batch_capture(ibx_batch* b, int use_graph);
-/* Capture `nslots` graph instances of the same step, each of which also records HIP events
- * (on the context stream) at the start, after the interpolation, before and after the tile
- * spread kernel, and at the end.  ibx_batch_run then replays instance (run count % nslots),
- * so nslots consecutive runs leave one timing record each. */
+/* Timed runs: `nslots` sets of HIP events on the context stream (start, after the
+ * interpolation, before and after the tile spread kernel, end); ibx_batch_run then enqueues
+ * the step eagerly, recording run n into slot n % nslots, so that nslots consecutive runs
+ * leave one timing record each.  (A graph's event-record nodes cannot be read back as
+ * timings, so timed runs are not graph replays.) */
 #define IBX_TIMED_EV 5
 int ibx_batch_capture_timed(ibx_batch* b, int nslots);
 int ibx_batch_run(ibx_batch* b);

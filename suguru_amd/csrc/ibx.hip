@@ -67,6 +67,8 @@ struct ibx_ctx {
   int num_cus = 256;
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;           // second stream (default priority) for overlap
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string err;
   int* d_err = nullptr;
   // spread binning
@@ -88,10 +90,8 @@ struct ibx_batch {
   hipGraphExec_t exec = nullptr;
   bool use_graph = false;
   bool captured = false;
-  // timed slots: graph instances that also record IBX_TIMED_EV events each; run n replays
-  // slot n % nslots
-  std::vector<hipGraph_t> tgraphs;
-  std::vector<hipGraphExec_t> texecs;
+  bool overlap = false;  // interpolation on the side stream, concurrent with the spread
+  // timed slots: IBX_TIMED_EV events per slot; timed run n records into slot n % nslots
   std::vector<hipEvent_t> tev;
   long long runs = 0;
 };
@@ -331,6 +331,9 @@ int ibx_create(ibx_ctx** out, const ibx_grid* g, int device) {
   d.dveul = g->dx * g->dy * g->dz;  // interp_spread.f90:534
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e == hipSuccess) e = hipMalloc(&c->d_err, sizeof(int));
   if (e == hipSuccess) e = hipMemsetAsync(c->d_err, 0, sizeof(int), c->own);
   if (e == hipSuccess) e = hipStreamSynchronize(c->own);
@@ -358,6 +361,10 @@ void ibx_destroy(ibx_ctx* c) {
                    &c->s_a3,   &c->s_a4, &c->s_o1,  &c->s_o2,    &c->s_o3, &c->s_off};
   for (DevBuf* b : all) release(*b);
   if (c->d_err) (void)hipFree(c->d_err);
+  if (c->side) (void)hipStreamSynchronize(c->side);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
 }
@@ -626,26 +633,54 @@ int ibx_batch_bind(ibx_batch* b, const ibx_batch_bufs* bufs) {
 
 // ev (optional): IBX_TIMED_EV events recorded at the start, after the interpolation, before
 // and after the tile spread kernel, and at the end
+static void drop_graph(ibx_batch* b);
+
+// With overlap, the interpolation runs on the side stream, forked from and joined back to
+// the context stream (the two stages share only read-only inputs); a captured graph then
+// holds two independent branches.
 static int batch_enqueue(ibx_batch* b, bool size_entries, const hipEvent_t* ev = nullptr) {
   ibx_ctx* c = b->ctx;
   const ibx_batch_bufs& q = b->bufs;
+  const bool ov = b->overlap && !size_entries;
   if (ev) HIPCHK(c, hipEventRecord(ev[0], c->stream));
-  int rc = enqueue_interp(c, q.u, q.v, q.w, b->npts, q.x, q.y, q.z, q.ul, q.vl, q.wl);
-  if (rc) return rc;
-  if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+  int rc;
+  if (ov) {
+    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    hipStream_t keep = c->stream;
+    c->stream = c->side;
+    rc = enqueue_interp(c, q.u, q.v, q.w, b->npts, q.x, q.y, q.z, q.ul, q.vl, q.wl);
+    if (!rc && ev) {
+      hipError_t e = hipEventRecord(ev[1], c->side);
+      if (e != hipSuccess) rc = fail(c, IBX_E_HIP, "hipEventRecord(side)");
+    }
+    c->stream = keep;
+    if (rc) return rc;
+    HIPCHK(c, hipEventRecord(c->ev_join, c->side));
+  } else {
+    rc = enqueue_interp(c, q.u, q.v, q.w, b->npts, q.x, q.y, q.z, q.ul, q.vl, q.wl);
+    if (rc) return rc;
+    if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
+  }
   rc = enqueue_spread(c, b->cfg, false, b->ncap, (const long long*)b->d_off.p, b->npts, q.x,
                       q.y, q.z, q.fx, q.fy, q.fz, q.area, q.forcex, q.forcey, q.forcez,
                       size_entries, ev ? ev[2] : nullptr, nullptr, ev ? ev[3] : nullptr);
   if (rc) return rc;
+  if (ov) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
   if (ev) HIPCHK(c, hipEventRecord(ev[4], c->stream));
   return IBX_OK;
 }
 
+int ibx_batch_set_overlap(ibx_batch* b, int on) {
+  if (!b) return IBX_E_ARG;
+  b->overlap = on != 0;
+  drop_graph(b);
+  return IBX_OK;
+}
+
 static void drop_graph(ibx_batch* b) {
-  for (auto e : b->texecs) (void)hipGraphExecDestroy(e);
-  for (auto g : b->tgraphs) (void)hipGraphDestroy(g);
   for (auto e : b->tev) (void)hipEventDestroy(e);
-  b->texecs.clear(), b->tgraphs.clear(), b->tev.clear();
+  b->tev.clear();
   b->runs = 0;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
@@ -704,16 +739,12 @@ static int capture_common(ibx_batch* b, int use_graph, int nslots) {
       c->ecap = c->entries.bytes / sizeof(unsigned long long);
     }
   }
-  b->use_graph = use_graph != 0 || nslots > 0;
+  b->use_graph = nslots == 0 && use_graph != 0;
   if (nslots > 0) {
+    // timing events cannot be read back from a graph's event-record nodes on this HIP, so
+    // timed runs are eager launches with events between the stages
     b->tev.assign((size_t)nslots * IBX_TIMED_EV, nullptr);
     for (auto& e : b->tev) HIPCHK(c, hipEventCreate(&e));
-    b->tgraphs.assign(nslots, nullptr);
-    b->texecs.assign(nslots, nullptr);
-    for (int sl = 0; sl < nslots; ++sl)
-      if ((rc = capture_graph(b, &b->tev[(size_t)sl * IBX_TIMED_EV], &b->tgraphs[sl],
-                              &b->texecs[sl])))
-        return rc;
   } else if (b->use_graph) {
     if ((rc = capture_graph(b, nullptr, &b->graph, &b->exec))) return rc;
   }
@@ -734,11 +765,11 @@ int ibx_batch_run(ibx_batch* b) {
   ibx_ctx* c = b->ctx;
   if (!b->captured) return fail(c, IBX_E_STATE, "batch not captured");
   HIPCHK(c, hipSetDevice(c->device));
-  if (!b->texecs.empty()) {
-    const size_t sl = (size_t)(b->runs % (long long)b->texecs.size());
-    HIPCHK(c, hipGraphLaunch(b->texecs[sl], c->stream));
+  if (!b->tev.empty()) {
+    const size_t ns = b->tev.size() / IBX_TIMED_EV;
+    const size_t sl = (size_t)(b->runs % (long long)ns);
     ++b->runs;
-    return IBX_OK;
+    return batch_enqueue(b, false, &b->tev[sl * IBX_TIMED_EV]);
   }
   ++b->runs;
   if (b->use_graph) {
@@ -760,15 +791,17 @@ int ibx_batch_reset_slots(ibx_batch* b) {
 int ibx_batch_slot_times(ibx_batch* b, double* us) {
   if (!b || !us) return IBX_E_ARG;
   ibx_ctx* c = b->ctx;
-  if (b->texecs.empty()) return fail(c, IBX_E_STATE, "batch not captured with timed slots");
+  if (b->tev.empty()) return fail(c, IBX_E_STATE, "batch not captured with timed slots");
   HIPCHK(c, hipSetDevice(c->device));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  const size_t ns = b->texecs.size();
+  const size_t ns = b->tev.size() / IBX_TIMED_EV;
   for (size_t sl = 0; sl < ns; ++sl) {
     const hipEvent_t* e = &b->tev[sl * IBX_TIMED_EV];
     float t[4];
     HIPCHK(c, hipEventElapsedTime(&t[0], e[0], e[1]));  // interpolation
-    HIPCHK(c, hipEventElapsedTime(&t[1], e[1], e[2]));  // binning, scan, wall planes
+    // binning, scan, wall planes: from the step start (with overlap the interpolation runs
+    // concurrently on the side stream)
+    HIPCHK(c, hipEventElapsedTime(&t[1], b->overlap ? e[0] : e[1], e[2]));
     HIPCHK(c, hipEventElapsedTime(&t[2], e[2], e[3]));  // tile spread kernel
     HIPCHK(c, hipEventElapsedTime(&t[3], e[0], e[4]));  // whole step
     for (int i = 0; i < 4; ++i) us[sl * 4 + i] = 1000.0 * (double)t[i];

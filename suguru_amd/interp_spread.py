@@ -208,6 +208,10 @@ class Batch:
     def set_config(self, i: int):
         self.ctx._check(self.lib.ibx_batch_set_config(self.h, int(i)))
 
+    def set_overlap(self, on: bool):
+        """Interpolation on a second stream, concurrent with the spread."""
+        self.ctx._check(self.lib.ibx_batch_set_overlap(self.h, 1 if on else 0))
+
     @staticmethod
     def config_names():
         lib = _lib.load()
@@ -217,8 +221,8 @@ class Batch:
         self.ctx._check(self.lib.ibx_batch_capture(self.h, 1 if use_graph else 0))
 
     def capture_timed(self, nslots: int):
-        """Capture `nslots` graph instances that also record HIP events around each stage;
-        run() cycles through them (ibx_batch_capture_timed)."""
+        """Timed runs: run() enqueues the step eagerly with HIP events around each stage, run n
+        recording into slot n % nslots (ibx_batch_capture_timed)."""
         self.ctx._check(self.lib.ibx_batch_capture_timed(self.h, int(nslots)))
         self._nslots = int(nslots)
 

@@ -134,6 +134,7 @@ class Surrogate:
         std = X.std(0).cpu().numpy()
         scale = np.where(std > 1e-12, 1.0 / np.maximum(std, 1e-12), 0.0).astype(np.float32)
         self.out_scale = float(Y.std().item())
+        self._norm = (mean.astype(np.float32), scale)
         self.mlp.set_norm(mean, scale, self.out_scale)
 
     def train(self, X, Y, steps, batch=512, lr=1e-3, lr_min=1e-5, seed=0, log_every=0):
@@ -215,3 +216,82 @@ class Surrogate:
             fv = f.view(3, ncap, self.npts)
             fv[:, idx] = fl.view(3, nb, self.npts)
         return f, area, bad
+
+    # ---- persistence ---------------------------------------------------------------------------
+    def save(self, path):
+        """Weights, biases and normalisation as an .npz (plain arrays, no pickles)."""
+        arrs = {}
+        for i in range(self.mlp.nl):
+            W, b = self.mlp.get_layer(i)
+            arrs[f"W{i}"], arrs[f"b{i}"] = W, b
+        arrs["norm_mean"], arrs["norm_scale"] = self._norm
+        arrs["out_scale"] = np.array([self.out_scale], np.float32)
+        arrs["meta"] = np.array([self.L, self.mlp.width, self.mlp.nhidden], np.int64)
+        np.savez(path, **arrs)
+
+    def load(self, path):
+        z = np.load(path, allow_pickle=False)
+        for i in range(self.mlp.nl):
+            self.mlp.set_layer(i, z[f"W{i}"], z[f"b{i}"])
+        self.out_scale = float(z["out_scale"][0])
+        self._norm = (z["norm_mean"], z["norm_scale"])
+        self.mlp.set_norm(z["norm_mean"], z["norm_scale"], self.out_scale)
+
+
+def run_experiment(n_train=200_000, n_test=2048, steps=20_000, batch=512, lr=1e-3, seed=1,
+                   tol=1e-3, width=512, nhidden=4, log_every=0, save=None):
+    """Train the surrogate on GPU-law labels and measure what it gains: per-capsule relative
+    traction error (trained vs the untrained network), the equilibrium indicator, and the
+    step time of surrogate + fallback against the FP64 law on the held-out capsules."""
+    import torch
+
+    s = Surrogate(L=16, width=width, nhidden=nhidden, max_batch=max(batch, 1024))
+    t0 = time.perf_counter()
+    X, Y, _ = s.dataset(n_train, seed)
+    Xt, Yt, Pt = s.dataset(n_test, seed + 7919)
+    t_data = time.perf_counter() - t0
+    s.fit_norm(X, Y)
+    s.mlp.init(seed)
+    err0 = s.rel_error(s.predict(Xt), Yt)
+    losses = s.train(X, Y, steps, batch=batch, lr=lr, seed=seed, log_every=log_every)
+    pred = s.predict(Xt)
+    err = s.rel_error(pred, Yt)
+    x, y, z = (Pt[k].reshape(-1).contiguous() for k in range(3))
+    _, area = s.geometry(x, y, z)
+    ind = s.indicator(pred, Pt.permute(1, 0, 2), area)
+    # timing: FP64 law vs surrogate step (coefficients + MLP + geometry + indicator + fallback)
+    reps = 5
+    for _ in range(2):
+        s.law(x, y, z)
+        s.step(x, y, z, tol)
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    for _ in range(reps):
+        s.law(x, y, z)
+    torch.cuda.synchronize()
+    t_law = (time.perf_counter() - a) / reps
+    a = time.perf_counter()
+    for _ in range(reps):
+        f, _, bad = s.step(x, y, z, tol)
+    torch.cuda.synchronize()
+    t_sur = (time.perf_counter() - a) / reps
+    fl, _, _ = s.law(x, y, z)
+    final = s.rel_error(f.view(3, n_test, -1).permute(1, 0, 2).reshape(n_test, -1),
+                        fl.view(3, n_test, -1).permute(1, 0, 2).reshape(n_test, -1))
+    if save:
+        s.save(save)
+    q = lambda v, p: float(torch.quantile(v.double().cpu(), p))
+    return {
+        "n_train": n_train, "n_test": n_test, "steps": steps, "batch": batch,
+        "data_s": t_data, "train_s": s.timings.get("train_s"),
+        "law_s_per_capsule": s.timings.get("law_s_per_capsule"),
+        "loss_first": losses[0], "loss_last": float(np.mean(losses[-50:])),
+        "rel_err_untrained_median": q(err0, 0.5),
+        "rel_err_median": q(err, 0.5), "rel_err_p90": q(err, 0.9), "rel_err_max": float(err.max()),
+        "indicator_median": q(ind, 0.5),
+        "indicator_corr_with_error": float(np.corrcoef(ind.cpu().numpy(), err.cpu().numpy())[0, 1]),
+        "tol": tol, "fallback_fraction": float(bad.double().mean()),
+        "final_rel_err_max": float(final.max()),
+        "t_law_ms": 1e3 * t_law, "t_surrogate_step_ms": 1e3 * t_sur,
+        "speedup": t_law / t_sur,
+    }

@@ -107,6 +107,24 @@ def test_batch_every_config_graph_and_eager_bitexact(gpu_ctx_factory, moderate):
                     assert np.array_equal(bits(t[k].cpu().numpy()), w), (name, graph, rep, k)
     prof = b.profile(reps=3)
     assert prof["total_us"] > 0
+    # interpolation on the side stream, concurrent with the spread: same bits
+    b.set_config(0)
+    b.set_overlap(True)
+    for graph in (True, False):
+        print("case: overlap", "graph" if graph else "eager", flush=True)
+        b.capture(use_graph=graph)
+        for rep in range(2):
+            for k in ("ul", "vl", "wl", "forcex", "forcey", "forcez"):
+                t[k].fill_(np.nan)
+            torch.cuda.synchronize()
+            b.run()
+            ctx.sync()
+            torch.cuda.synchronize()
+            for k, w in zip(("ul", "vl", "wl"), want_i):
+                assert np.array_equal(bits(t[k].cpu().numpy()), w), ("overlap", graph, rep, k)
+            for k, w in zip(("forcex", "forcey", "forcez"), want_s):
+                assert np.array_equal(bits(t[k].cpu().numpy()), w), ("overlap", graph, rep, k)
+    b.set_overlap(False)
     # timed graph instances (the benchmark's timed region): same bits, one timing record each
     print("case: timed slots", flush=True)
     b.set_config(0)

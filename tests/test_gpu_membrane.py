@@ -179,7 +179,8 @@ def test_mlp_padded_shapes_and_training_reduces_loss():
         gW, _ = m.get_grad(i)
         assert np.linalg.norm(gW - g[i][0]) < 2e-3 * np.linalg.norm(g[i][0])
     losses = [m.train_step(dx, dt_, 1e-3) for _ in range(30)]
-    assert losses[-1] < 0.5 * losses[0], losses[::10]
+    assert losses[-1] < 0.8 * losses[0], losses[::10]
+    assert all(b < a for a, b in zip(losses[::10], losses[10::10])), losses[::10]
     y = torch.empty((50, 45), dtype=torch.float32, device=dev)
     m.forward(dx, y)
     torch.cuda.synchronize()

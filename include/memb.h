@@ -79,7 +79,7 @@ int memb_geometry_f64_dev(memb_plan* p, int64_t ncap, const double* x, const dou
 /* Layers: din -> w (SiLU) -> w (SiLU) -> ... (nhidden layers of width w) -> dout, linear
  * output.  Weights are FP32 masters; the GEMMs read BF16 copies, refreshed whenever the
  * masters change (set_layer, init, each Adam update).  Every GEMM dimension is padded
- * internally to a multiple of 64.  The loss is taken on the scaled outputs y / out_scale. */
+ * internally to a multiple of 64.  The loss is taken on the normalised outputs y'. */
 typedef struct memb_mlp memb_mlp;
 int memb_mlp_create(memb_mlp** out, int din, int width, int nhidden, int dout, int max_batch,
                     int device);
@@ -92,15 +92,17 @@ int memb_mlp_layer_shape(const memb_mlp* m, int i, int* out_dim, int* in_dim);
 /* copy FP32 weights/biases of layer i in or out (host pointers) */
 int memb_mlp_set_layer(memb_mlp* m, int i, const float* W, const float* b);
 int memb_mlp_get_layer(memb_mlp* m, int i, float* W, float* b);
-/* input normalisation x' = (x - mean) * scale (din values each) and output scale: the network
- * works on x' and predicts y / out_scale */
-int memb_mlp_set_norm(memb_mlp* m, const float* mean, const float* scale, float out_scale);
+/* input normalisation x' = (x - mean) * scale (din values each) and output de-normalisation
+ * y = out_mean + out_scale * y' (dout values each): the network maps x' to y' */
+int memb_mlp_set_norm(memb_mlp* m, const float* mean, const float* scale, const float* out_mean,
+                      const float* out_scale);
 /* seeded init: W ~ U(-a, a), a = sqrt(6 / (in + out)) (SiLU layers) ; b = 0 */
 int memb_mlp_init(memb_mlp* m, uint64_t seed);
 /* forward for `batch` rows: x[batch][din] FP32 (device) -> y[batch][dout] FP32 (device) */
 int memb_mlp_forward(memb_mlp* m, int batch, const float* x, float* y);
 /* one training step on a batch: forward, loss = mean over rows and outputs of
- * (y - t)^2 * wt (wt[dout] FP32 per-output weight, NULL = 1), backward, Adam update
+ * (y' - t')^2 * wt (normalised outputs and targets; wt[dout] FP32 per-output weight,
+ * NULL = 1), backward, Adam update
  * (lr, beta1 0.9, beta2 0.999, eps 1e-8).  *loss (host) receives the batch loss. */
 int memb_mlp_train_step(memb_mlp* m, int batch, const float* x, const float* t,
                         const float* wt, float lr, float* loss);

@@ -163,7 +163,7 @@ def load_memb() -> ctypes.CDLL:
         "memb_mlp_layer_shape": (I, [P, I, ctypes.POINTER(I), ctypes.POINTER(I)]),
         "memb_mlp_set_layer": (I, [P, I, P, P]),
         "memb_mlp_get_layer": (I, [P, I, P, P]),
-        "memb_mlp_set_norm": (I, [P, P, P, F]),
+        "memb_mlp_set_norm": (I, [P, P, P, P, P]),
         "memb_mlp_init": (I, [P, ctypes.c_uint64]),
         "memb_mlp_forward": (I, [P, I, P, P]),
         "memb_mlp_train_step": (I, [P, I, P, P, P, F, ctypes.POINTER(F)]),

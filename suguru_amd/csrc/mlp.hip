@@ -137,18 +137,21 @@ __global__ void k_input(int batch, int din, int Mp, int Dp, const float* __restr
   X[q] = to_bf16(v);
 }
 
-// y[batch][dout] = out_scale * Y (fp32)
-__global__ void k_output(int batch, int dout, int ldy, float out_scale, const float* __restrict__ Y,
+// y[batch][dout] = omean + oscale * Y (fp32), per output
+__global__ void k_output(int batch, int dout, int ldy, const float* __restrict__ omean,
+                         const float* __restrict__ oscale, const float* __restrict__ Y,
                          float* __restrict__ y) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (long long)batch * dout) return;
   const int r = (int)(q / dout), c = (int)(q % dout);
-  y[q] = out_scale * Y[(size_t)r * ldy + c];
+  y[q] = omean[c] + oscale[c] * Y[(size_t)r * ldy + c];
 }
 
-// loss = mean((Y - t/s)^2 w) over batch x dout; dY = bf16(2 (Y - t/s) w / (batch dout))
-__global__ void k_loss(int batch, int dout, int Mp, int ldy, float inv_scale,
-                       const float* __restrict__ Y, const float* __restrict__ t,
+// loss = mean((Y - t')^2 w) over batch x dout with t' = (t - omean) / oscale;
+// dY = bf16(2 (Y - t') w / (batch dout))
+__global__ void k_loss(int batch, int dout, int Mp, int ldy, const float* __restrict__ omean,
+                       const float* __restrict__ oscale, const float* __restrict__ Y,
+                       const float* __restrict__ t,
                        const float* __restrict__ wt, __bf16* __restrict__ dY,
                        float* __restrict__ loss) {
   __shared__ float red[256];
@@ -159,7 +162,7 @@ __global__ void k_loss(int batch, int dout, int Mp, int ldy, float inv_scale,
     float g = 0.0f;
     if (r < batch && c < dout) {
       const float w = wt ? wt[c] : 1.0f;
-      const float d = Y[q] - t[(size_t)r * dout + c] * inv_scale;
+      const float d = Y[q] - (t[(size_t)r * dout + c] - omean[c]) / oscale[c];
       const float inv = 1.0f / ((float)batch * (float)dout);
       l = d * d * w * inv;
       g = 2.0f * d * w * inv;
@@ -237,8 +240,8 @@ struct memb_mlp {
   __bf16* dZa = nullptr;           // [Mp][Wp] (two buffers, ping-pong)
   __bf16* dZb = nullptr;
   float* loss = nullptr;
-  float *mean = nullptr, *scale = nullptr;
-  float out_scale = 1.0f;
+  float *mean = nullptr, *scale = nullptr;    // input normalisation [Dp]
+  float *omean = nullptr, *oscale = nullptr;  // output de-normalisation [Op]
   long long step = 0;
   std::vector<void*> allocs;
 };
@@ -303,7 +306,7 @@ int backward(memb_mlp* m, int batch, const float* x, const float* t, const float
   const int Mp = pad64(batch);
   k_zero<<<1, 64, 0, m->stream>>>(m->loss, 1);
   k_loss<<<nblk((long long)Mp * m->Op), 256, 0, m->stream>>>(batch, m->dout, Mp, m->Op,
-                                                            1.0f / m->out_scale, m->Y, t, wt,
+                                                            m->omean, m->oscale, m->Y, t, wt,
                                                             m->dY, m->loss);
   const __bf16* dcur = m->dY;  // gradient w.r.t. the current layer's pre-activation
   int ldd = m->Op;
@@ -379,10 +382,12 @@ int memb_mlp_create(memb_mlp** out, int din, int width, int nhidden, int dout, i
       (rc = dalloc(m, &m->dY, (size_t)m->Mp * m->Op)) ||
       (rc = dalloc(m, &m->dZa, (size_t)m->Mp * m->Wp)) ||
       (rc = dalloc(m, &m->dZb, (size_t)m->Mp * m->Wp)) || (rc = dalloc(m, &m->loss, 1)) ||
-      (rc = dalloc(m, &m->mean, (size_t)m->Dp)) || (rc = dalloc(m, &m->scale, (size_t)m->Dp)))
+      (rc = dalloc(m, &m->mean, (size_t)m->Dp)) || (rc = dalloc(m, &m->scale, (size_t)m->Dp)) ||
+      (rc = dalloc(m, &m->omean, (size_t)m->Op)) || (rc = dalloc(m, &m->oscale, (size_t)m->Op)))
     return rc;
-  std::vector<float> one(m->Dp, 1.0f);
-  XCHK(m, hipMemcpy(m->scale, one.data(), one.size() * sizeof(float), hipMemcpyHostToDevice));
+  std::vector<float> one(std::max(m->Dp, m->Op), 1.0f);
+  XCHK(m, hipMemcpy(m->scale, one.data(), m->Dp * sizeof(float), hipMemcpyHostToDevice));
+  XCHK(m, hipMemcpy(m->oscale, one.data(), m->Op * sizeof(float), hipMemcpyHostToDevice));
   return MEMB_OK;
 }
 
@@ -412,12 +417,16 @@ int memb_mlp_layer_shape(const memb_mlp* m, int i, int* out_dim, int* in_dim) {
   return MEMB_OK;
 }
 
-int memb_mlp_set_norm(memb_mlp* m, const float* mean, const float* scale, float out_scale) {
-  if (!m || !mean || !scale || !(out_scale > 0.0f)) return MEMB_E_ARG;
+int memb_mlp_set_norm(memb_mlp* m, const float* mean, const float* scale, const float* out_mean,
+                      const float* out_scale) {
+  if (!m || !mean || !scale || !out_mean || !out_scale) return MEMB_E_ARG;
+  for (int c = 0; c < m->dout; ++c)
+    if (!(out_scale[c] > 0.0f)) return mfail(m, MEMB_E_ARG, "out_scale must be positive");
   XCHK(m, hipSetDevice(m->device));
   XCHK(m, hipMemcpy(m->mean, mean, m->din * sizeof(float), hipMemcpyHostToDevice));
   XCHK(m, hipMemcpy(m->scale, scale, m->din * sizeof(float), hipMemcpyHostToDevice));
-  m->out_scale = out_scale;
+  XCHK(m, hipMemcpy(m->omean, out_mean, m->dout * sizeof(float), hipMemcpyHostToDevice));
+  XCHK(m, hipMemcpy(m->oscale, out_scale, m->dout * sizeof(float), hipMemcpyHostToDevice));
   return MEMB_OK;
 }
 
@@ -484,7 +493,7 @@ int memb_mlp_forward(memb_mlp* m, int batch, const float* x, float* y) {
   int rc = forward(m, batch, x);
   if (rc) return rc;
   k_output<<<nblk((long long)batch * m->dout), 256, 0, m->stream>>>(batch, m->dout, m->Op,
-                                                                   m->out_scale, m->Y, y);
+                                                                   m->omean, m->oscale, m->Y, y);
   XCHK(m, hipGetLastError());
   return MEMB_OK;
 }

@@ -110,12 +110,12 @@ class SurrogateMLP:
     def init(self, seed):
         self._check(self.lib.memb_mlp_init(self.h, int(seed)))
 
-    def set_norm(self, mean, scale, out_scale):
-        mean = np.ascontiguousarray(mean, dtype=np.float32)
-        scale = np.ascontiguousarray(scale, dtype=np.float32)
-        self._check(self.lib.memb_mlp_set_norm(self.h, mean.ctypes.data_as(ctypes.c_void_p),
-                                               scale.ctypes.data_as(ctypes.c_void_p),
-                                               float(out_scale)))
+    def set_norm(self, mean, scale, out_mean, out_scale):
+        """x' = (x - mean) * scale; y = out_mean + out_scale * y' (scalars broadcast)."""
+        f = lambda a, n: np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float32), (n,)))
+        arrs = [f(mean, self.din), f(scale, self.din), f(out_mean, self.dout), f(out_scale, self.dout)]
+        self._norm_arrays = arrs
+        self._check(self.lib.memb_mlp_set_norm(self.h, *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs]))
 
     def set_layer(self, i, W, b):
         W = np.ascontiguousarray(W, dtype=np.float32)

@@ -80,7 +80,6 @@ class Surrogate:
         self.mlp = SurrogateMLP(self.din, width, nhidden, self.dout, max_batch, device)
         self.max_batch = max_batch
         self.gs, self.eb = gs, eb
-        self.out_scale = 1.0
         self.timings = {}
         # one stream for the library kernels and torch's own work (indexing, copies)
         s = torch.cuda.current_stream(self.dev).cuda_stream
@@ -106,7 +105,7 @@ class Surrogate:
         self.plan.coeffs_dev(x, y, z, c)
         return c.view(ncap, self.din)
 
-    def dataset(self, n, seed, chunk=8192):
+    def dataset(self, n, seed, chunk=8192, amp_max=0.3):
         """(inputs fp32 [n, din], targets fp32 [n, dout], positions fp64 [3, n, npts])."""
         t = self.torch
         X = t.empty((n, self.din), dtype=t.float32, device=self.dev)
@@ -115,7 +114,7 @@ class Surrogate:
         t_law = 0.0
         for a in range(0, n, chunk):
             b = min(n, a + chunk)
-            xs, ys, zs = random_shapes(self.L, b - a, seed * 1000003 + a)
+            xs, ys, zs = random_shapes(self.L, b - a, seed * 1000003 + a, amp_max=amp_max)
             x, y, z = self.upload(xs), self.upload(ys), self.upload(zs)
             X[a:b] = self.coeffs(x, y, z).float()
             t.cuda.synchronize()
@@ -130,12 +129,15 @@ class Surrogate:
 
     # ---- training ----------------------------------------------------------------------------
     def fit_norm(self, X, Y):
-        mean = X.mean(0).cpu().numpy()
+        """Per-feature input standardisation and per-output target standardisation."""
+        mean = X.mean(0).cpu().numpy().astype(np.float32)
         std = X.std(0).cpu().numpy()
         scale = np.where(std > 1e-12, 1.0 / np.maximum(std, 1e-12), 0.0).astype(np.float32)
-        self.out_scale = float(Y.std().item())
-        self._norm = (mean.astype(np.float32), scale)
-        self.mlp.set_norm(mean, scale, self.out_scale)
+        omean = Y.mean(0).cpu().numpy().astype(np.float32)
+        ostd = Y.std(0).cpu().numpy().astype(np.float32)
+        ostd = np.maximum(ostd, 1e-6 * float(ostd.max())).astype(np.float32)
+        self._norm = (mean, scale, omean, ostd)
+        self.mlp.set_norm(mean, scale, omean, ostd)
 
     def train(self, X, Y, steps, batch=512, lr=1e-3, lr_min=1e-5, seed=0, log_every=0):
         t = self.torch
@@ -224,8 +226,7 @@ class Surrogate:
         for i in range(self.mlp.nl):
             W, b = self.mlp.get_layer(i)
             arrs[f"W{i}"], arrs[f"b{i}"] = W, b
-        arrs["norm_mean"], arrs["norm_scale"] = self._norm
-        arrs["out_scale"] = np.array([self.out_scale], np.float32)
+        arrs["norm_mean"], arrs["norm_scale"], arrs["out_mean"], arrs["out_scale"] = self._norm
         arrs["meta"] = np.array([self.L, self.mlp.width, self.mlp.nhidden], np.int64)
         np.savez(path, **arrs)
 
@@ -233,13 +234,12 @@ class Surrogate:
         z = np.load(path, allow_pickle=False)
         for i in range(self.mlp.nl):
             self.mlp.set_layer(i, z[f"W{i}"], z[f"b{i}"])
-        self.out_scale = float(z["out_scale"][0])
-        self._norm = (z["norm_mean"], z["norm_scale"])
-        self.mlp.set_norm(z["norm_mean"], z["norm_scale"], self.out_scale)
+        self._norm = (z["norm_mean"], z["norm_scale"], z["out_mean"], z["out_scale"])
+        self.mlp.set_norm(*self._norm)
 
 
 def run_experiment(n_train=200_000, n_test=2048, steps=20_000, batch=512, lr=1e-3, seed=1,
-                   tol=1e-3, width=512, nhidden=4, log_every=0, save=None):
+                   tol=1e-3, width=512, nhidden=4, log_every=0, save=None, amp_max=0.3):
     """Train the surrogate on GPU-law labels and measure what it gains: per-capsule relative
     traction error (trained vs the untrained network), the equilibrium indicator, and the
     step time of surrogate + fallback against the FP64 law on the held-out capsules."""
@@ -247,8 +247,8 @@ def run_experiment(n_train=200_000, n_test=2048, steps=20_000, batch=512, lr=1e-
 
     s = Surrogate(L=16, width=width, nhidden=nhidden, max_batch=max(batch, 1024))
     t0 = time.perf_counter()
-    X, Y, _ = s.dataset(n_train, seed)
-    Xt, Yt, Pt = s.dataset(n_test, seed + 7919)
+    X, Y, _ = s.dataset(n_train, seed, amp_max=amp_max)
+    Xt, Yt, Pt = s.dataset(n_test, seed + 7919, amp_max=amp_max)
     t_data = time.perf_counter() - t0
     s.fit_norm(X, Y)
     s.mlp.init(seed)
@@ -282,7 +282,8 @@ def run_experiment(n_train=200_000, n_test=2048, steps=20_000, batch=512, lr=1e-
         s.save(save)
     q = lambda v, p: float(torch.quantile(v.double().cpu(), p))
     return {
-        "n_train": n_train, "n_test": n_test, "steps": steps, "batch": batch,
+        "n_train": n_train, "n_test": n_test, "steps": steps, "batch": batch, "lr": lr,
+        "amp_max": amp_max,
         "data_s": t_data, "train_s": s.timings.get("train_s"),
         "law_s_per_capsule": s.timings.get("law_s_per_capsule"),
         "loss_first": losses[0], "loss_last": float(np.mean(losses[-50:])),

@@ -19,9 +19,10 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--log-every", type=int, default=1000)
     ap.add_argument("--save", default=None)
+    ap.add_argument("--amp-max", type=float, default=0.3)
     a = ap.parse_args()
     r = run_experiment(a.n_train, a.n_test, a.steps, a.batch, a.lr, a.seed, a.tol,
-                       log_every=a.log_every, save=a.save)
+                       log_every=a.log_every, save=a.save, amp_max=a.amp_max)
     print(json.dumps(r), flush=True)
 
 

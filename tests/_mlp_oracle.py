@@ -31,10 +31,12 @@ def dsilu(z):
 
 
 class MLPOracle:
-    def __init__(self, layers, mean, scale, out_scale, exact=False):
-        """layers: list of (W [out x in], b [out]) in float32."""
+    def __init__(self, layers, mean, scale, out_scale, exact=False, out_mean=0.0):
+        """layers: list of (W [out x in], b [out]) in float32; out_scale / out_mean: scalars or
+        per-output arrays."""
         self.layers = layers
         self.mean, self.scale, self.out_scale = mean, scale, out_scale
+        self.out_mean = out_mean
         self.exact = exact
 
     def _r(self, a):
@@ -60,7 +62,7 @@ class MLPOracle:
 
     def predict(self, x):
         Zs, _ = self.forward(x)
-        return self.out_scale * Zs[-1]
+        return self.out_mean + self.out_scale * Zs[-1]
 
     def grads(self, x, t, wt=None):
         dt = np.float64 if self.exact else np.float32
@@ -68,7 +70,7 @@ class MLPOracle:
         Y = Zs[-1]
         B, D = Y.shape
         w = np.ones(D, dt) if wt is None else np.asarray(wt, dt)
-        d = Y - np.asarray(t, dt) / dt(self.out_scale)
+        d = Y - ((np.asarray(t, dt) - np.asarray(self.out_mean, dt)) / np.asarray(self.out_scale, dt)).astype(dt)
         loss = float(np.sum(d * d * w) / (B * D))
         g = self._r((2.0 * d * w / (B * D)).astype(dt))
         out = [None] * len(self.layers)

@@ -136,9 +136,9 @@ def test_mlp_forward_and_gradients_match_oracle():
     m = SurrogateMLP(din, 512, 4, dout, max_batch=256)
     m.set_stream(torch.cuda.current_stream().cuda_stream)
     m.init(7)
-    m.set_norm(mean, scale, 3.0)
+    m.set_norm(mean, scale, 0.5, 3.0)
     layers = [m.get_layer(i) for i in range(m.nl)]
-    ref = MLPOracle(layers, mean, scale, 3.0)
+    ref = MLPOracle(layers, mean, scale, 3.0, out_mean=0.5)
     dev = "cuda:0"
     dx, dt_, dw = (torch.from_numpy(a).to(dev) for a in (x, t, wt))
     y = torch.empty((batch, dout), dtype=torch.float32, device=dev)
@@ -168,12 +168,12 @@ def test_mlp_padded_shapes_and_training_reduces_loss():
     m = SurrogateMLP(37, 70, 2, 45, max_batch=64)
     m.set_stream(torch.cuda.current_stream().cuda_stream)
     m.init(3)
-    m.set_norm(mean, scale, 3.0)
+    m.set_norm(mean, scale, 0.5, 3.0)
     dev = "cuda:0"
     dx, dt_ = torch.from_numpy(x).to(dev), torch.from_numpy(t).to(dev)
     l0 = m.grad_only(dx, dt_)
     layers = [m.get_layer(i) for i in range(m.nl)]
-    lr_, g = MLPOracle(layers, mean, scale, 3.0).grads(x, t)
+    lr_, g = MLPOracle(layers, mean, scale, 3.0, out_mean=0.5).grads(x, t)
     assert abs(l0 - lr_) < 2e-3 * lr_
     for i in range(m.nl):
         gW, _ = m.get_grad(i)
@@ -184,6 +184,6 @@ def test_mlp_padded_shapes_and_training_reduces_loss():
     y = torch.empty((50, 45), dtype=torch.float32, device=dev)
     m.forward(dx, y)
     torch.cuda.synchronize()
-    ref = MLPOracle([m.get_layer(i) for i in range(m.nl)], mean, scale, 3.0).predict(x)
+    ref = MLPOracle([m.get_layer(i) for i in range(m.nl)], mean, scale, 3.0, out_mean=0.5).predict(x)
     assert np.linalg.norm(y.cpu().numpy() - ref) < 3e-3 * np.linalg.norm(ref)
     m.close()

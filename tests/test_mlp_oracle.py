@@ -24,7 +24,7 @@ def test_backward_matches_finite_differences():
               (rng.normal(size=(dout, w)) * 0.5, rng.normal(size=dout) * 0.1)]
     mean, scale, s = rng.normal(size=din), rng.uniform(0.5, 2, din), 3.0
     x, t, wt = rng.normal(size=(B, din)), rng.normal(size=(B, dout)), rng.uniform(0.5, 1.5, dout)
-    o = MLPOracle(layers, mean, scale, s, exact=True)
+    o = MLPOracle(layers, mean, scale, s, exact=True, out_mean=0.3)
     loss, g = o.grads(x, t, wt)
     eps = 1e-6
     for i, (W, b) in enumerate(layers):
@@ -33,15 +33,15 @@ def test_backward_matches_finite_differences():
             Wp[r, c] += eps
             Wm[r, c] -= eps
             lp = MLPOracle([(Wp, b) if k == i else layers[k] for k in range(3)], mean, scale, s,
-                           exact=True).grads(x, t, wt)[0]
+                           exact=True, out_mean=0.3).grads(x, t, wt)[0]
             lm = MLPOracle([(Wm, b) if k == i else layers[k] for k in range(3)], mean, scale, s,
-                           exact=True).grads(x, t, wt)[0]
+                           exact=True, out_mean=0.3).grads(x, t, wt)[0]
             assert abs((lp - lm) / (2 * eps) - g[i][0][r, c]) < 1e-8 * max(1.0, abs(g[i][0][r, c]))
         bp, bm = b.copy(), b.copy()
         bp[0] += eps
         bm[0] -= eps
         lp = MLPOracle([(W, bp) if k == i else layers[k] for k in range(3)], mean, scale, s,
-                       exact=True).grads(x, t, wt)[0]
+                       exact=True, out_mean=0.3).grads(x, t, wt)[0]
         lm = MLPOracle([(W, bm) if k == i else layers[k] for k in range(3)], mean, scale, s,
-                       exact=True).grads(x, t, wt)[0]
+                       exact=True, out_mean=0.3).grads(x, t, wt)[0]
         assert abs((lp - lm) / (2 * eps) - g[i][1][0]) < 1e-8 * max(1.0, abs(g[i][1][0]))

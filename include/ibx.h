@@ -194,13 +194,18 @@ int ibx_batch_run(ibx_batch* b);
 /* Wait for the stream and restart the run count at slot 0. */
 int ibx_batch_reset_slots(ibx_batch* b);
 /* Wait for the stream; us[4*s + i] receives slot s's last replay, in microseconds:
- * [0] interpolation, [1] binning + scan + wall planes, [2] tile spread kernel, [3] whole step. */
+ * [0] interpolation, [1] spread set-up (tile configurations: binning + scan + wall planes;
+ * object configurations: bitmap clear, object boxes and items, sources), [2] the spread's
+ * main kernel (tile kernel, or the persistent item / zero-unit kernel), [3] whole step. */
 int ibx_batch_slot_times(ibx_batch* b, double* us);
 /* Run interp + spread `reps` times eagerly with HIP events on the context stream around each
- * stage; us[0..IBX_PROF_N) receives, in microseconds: [0] interpolation kernel, [1] binning
- * (count, scan, fill), [2] tile spread kernel, [3] whole sequence, [4] tile spread kernel's
- * tiles with candidates and [5] its tiles without candidates when run as two launches; and the work counts of the
- * last run: [6] non-empty tiles, [7] tiles, [8] candidate entries. */
+ * stage; us[0..IBX_PROF_N) receives, in microseconds: [0] interpolation kernel, [1] spread
+ * set-up (as for ibx_batch_slot_times), [2] the spread's main kernel, [3] whole sequence,
+ * [4] tile configurations: the tile kernel's tiles with candidates, objects: the main kernel,
+ * [5] tile configurations: tiles without candidates, objects: the y-halo copy; and the work
+ * counts of the last run: tile configurations [6] non-empty tiles, [7] tiles, [8] candidate
+ * entries; object configurations [6] work items, [7] extra source objects (sum over items of
+ * their source objects minus one), [8] 0. */
 #define IBX_PROF_N 9
 int ibx_batch_profile(ibx_batch* b, int reps, double* us);
 void ibx_batch_destroy(ibx_batch* b);

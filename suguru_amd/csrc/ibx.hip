@@ -78,7 +78,7 @@ struct ibx_ctx {
   // spread binning
   DevBuf tcount, toff, tpart, tlist, tmeta, entries;
   // spread by capsule objects
-  DevBuf oboxes, oubox, oinfo, oitems, ocl, oclcnt, otouch, oowner, ofixc, ofixo, ofix3, ometa;
+  DevBuf oboxes, oubox, oitems, osrc, onsrc, ocover, ometa;
   // staging for the host-array entry points
   DevBuf s_f1, s_f2, s_f3, s_x, s_y, s_z, s_a1, s_a2, s_a3, s_a4, s_o1, s_o2, s_o3, s_off;
   int cfg = 0;
@@ -185,50 +185,51 @@ int enqueue_scan(ibx_ctx* c, const unsigned* counts, long long n, unsigned* off)
   return IBX_OK;
 }
 
-constexpr int kFixCap = 1 << 20;  // cells touched by several capsules, per spread
-
-// The spread by capsule objects on the context stream (zero, boxes and items, object
-// kernel, fix-up of multi-object cells, x then y halos).  ev_tile / ev_tile_end (optional)
-// bracket the object kernel.
+// The spread by capsule objects on the context stream: bitmap and counters, object boxes
+// and items, sources and bitmap, the persistent item / zero-unit kernel, y halos (x halos
+// are written with the cells).  ev_tile / ev_tile_end (optional) bracket the persistent
+// kernel; ev_split is recorded with ev_tile_end.
 int enqueue_spread_obj(ibx_ctx* c, const TileCfg& cf, bool cc, long long ncap,
                        const long long* d_off, long long npts, const double* x,
                        const double* y, const double* z, const double* f1, const double* f2,
                        const double* f3, const double* area, double* o1, double* o2, double* o3,
                        hipEvent_t ev_tile, hipEvent_t ev_tile_end, hipEvent_t ev_split) {
+  if (npts >= OB_MAXPTS) return fail(c, IBX_E_ARG, "object spread: more than 2^27 points");
   const int klo = cc ? 0 : 1, khi = cc ? c->grid.ktot + 1 : c->grid.ktot;
   const long long ncells = c->gd.sz * (long long)(c->grid.ktot + 2);
+  const long long nrows = (long long)(c->grid.jtot + 2) * (c->grid.ktot + 2);
   const long long icap = std::max<long long>(1024, 64 * ncap + 1024);
-  const long long clcap = std::min<long long>((1LL << 31) - 1, 48 * npts + (1LL << 20));
-  const long long n4 = (ncells + 3) / 4;
+  const long long nw = (ncells + 31) / 32;
   const long long nobj = std::max<long long>(1, 9 * ncap);
   int rc;
   if ((rc = ensure(c, c->oboxes, (size_t)nobj * 6 * sizeof(int))) ||
       (rc = ensure(c, c->oubox, (size_t)std::max<long long>(1, ncap) * 6 * sizeof(int))) ||
-      (rc = ensure(c, c->oinfo, (size_t)nobj * OB_INFO * sizeof(int))) ||
       (rc = ensure(c, c->oitems, (size_t)icap * sizeof(ObjItem))) ||
-      (rc = ensure(c, c->ocl, (size_t)clcap * sizeof(int))) ||
-      (rc = ensure(c, c->oclcnt, (size_t)icap * sizeof(int))) ||
-      (rc = ensure(c, c->otouch, (size_t)n4 * sizeof(unsigned))) ||
-      (rc = ensure(c, c->oowner, (size_t)ncells * sizeof(int))) ||
-      (rc = ensure(c, c->ofixc, (size_t)kFixCap * sizeof(int))) ||
-      (rc = ensure(c, c->ofixo, (size_t)kFixCap * sizeof(int))) ||
-      (rc = ensure(c, c->ofix3, (size_t)kFixCap * sizeof(int))) ||
+      (rc = ensure(c, c->osrc, (size_t)icap * OB_NSRC * sizeof(int))) ||
+      (rc = ensure(c, c->onsrc, (size_t)icap * sizeof(int))) ||
+      (rc = ensure(c, c->ocover, (size_t)nw * sizeof(unsigned))) ||
       (rc = ensure(c, c->ometa, 8 * sizeof(int))))
     return rc;
   int* meta = (int*)c->ometa.p;
-  k_obj_zero<<<blocks_for((ncells + 1) / 2, 256), 256, 0, c->stream>>>(
-      ncells, o1, o2, o3, n4, (unsigned*)c->otouch.p, meta);
-  if (npts > 0 && ncap > 0)
+  unsigned* cover = (unsigned*)c->ocover.p;
+  k_obj_clear<<<blocks_for(std::max<long long>(nw, 8), 256), 256, 0, c->stream>>>(nw, cover,
+                                                                                 meta);
+  if (npts > 0 && ncap > 0) {
     k_obj_boxes<<<(unsigned)((ncap + 3) / 4), 256, 0, c->stream>>>(
         c->gd, klo, khi, ncap, d_off, x, y, z, (int*)c->oboxes.p, (int*)c->oubox.p,
-        (int*)c->oinfo.p, (ObjItem*)c->oitems.p, (int)icap, clcap, meta, c->d_err);
+        (ObjItem*)c->oitems.p, (int)icap, meta, c->d_err);
+    k_obj_sources<<<(unsigned)(c->num_cus * 4), 256, 0, c->stream>>>(
+        c->gd, ncap, (const int*)c->oboxes.p, (const int*)c->oubox.p,
+        (const ObjItem*)c->oitems.p, (int)icap, (int*)c->osrc.p, (int*)c->onsrc.p, cover, meta,
+        c->d_err);
+  }
   if (ev_tile) HIPCHK(c, hipEventRecord(ev_tile, c->stream));
-  if (npts > 0 && ncap > 0) {
+  {
     const unsigned nblk = (unsigned)((long long)c->num_cus * cf.wg_per_cu);
-#define OBJ_ARGS                                                                           \
+#define OBJ_ARGS                                                                             \
   c->gd, klo, khi, d_off, x, y, z, f1, f2, f3, area, (const ObjItem*)c->oitems.p, (int)icap, \
-      meta, (int*)c->ocl.p, (int*)c->oclcnt.p, (unsigned*)c->otouch.p, (int*)c->oowner.p,   \
-      (int*)c->ofixc.p, (int*)c->ofixo.p, (int*)c->ofix3.p, kFixCap, c->d_err, o1, o2, o3
+      (const int*)c->osrc.p, (const int*)c->onsrc.p, (const unsigned*)cover, nrows, meta, o1, \
+      o2, o3
     if (cc)
       k_obj_spread<true><<<nblk, OB_THREADS, 0, c->stream>>>(OBJ_ARGS);
     else
@@ -236,29 +237,7 @@ int enqueue_spread_obj(ibx_ctx* c, const TileCfg& cf, bool cc, long long ncap,
 #undef OBJ_ARGS
   }
   if (ev_tile_end) HIPCHK(c, hipEventRecord(ev_tile_end, c->stream));
-  // (timing split: object kernel | fix-ups and halos)
   if (ev_split) HIPCHK(c, hipEventRecord(ev_split, c->stream));
-  if (npts > 0 && ncap > 0) {
-    const unsigned nf = (unsigned)(c->num_cus * 8);  // 4 wavefronts (cells) per workgroup
-#define FIX2_ARGS                                                                             \
-  c->gd, klo, khi, (const int*)c->oinfo.p, (const ObjItem*)c->oitems.p, (const int*)c->ocl.p, \
-      (const int*)c->oclcnt.p, (const unsigned*)c->otouch.p, (const int*)c->oowner.p, meta,   \
-      (const int*)c->ofixc.p, (const int*)c->ofixo.p, kFixCap, x, y, z, f1, f2, f3, area, o1, \
-      o2, o3
-#define FIX3_ARGS                                                                           \
-  c->gd, klo, khi, ncap, d_off, x, y, z, f1, f2, f3, area, (const int*)c->oboxes.p,         \
-      (const int*)c->oubox.p, meta, (const int*)c->ofix3.p, kFixCap, o1, o2, o3
-    if (cc) {
-      k_obj_fixup2<true><<<nf, 256, 0, c->stream>>>(FIX2_ARGS);
-      k_obj_fixup3<true><<<(unsigned)c->num_cus, 256, 0, c->stream>>>(FIX3_ARGS);
-    } else {
-      k_obj_fixup2<false><<<nf, 256, 0, c->stream>>>(FIX2_ARGS);
-      k_obj_fixup3<false><<<(unsigned)c->num_cus, 256, 0, c->stream>>>(FIX3_ARGS);
-    }
-#undef FIX2_ARGS
-#undef FIX3_ARGS
-  }
-  // x halos were written with the cells (put_cell); y halos: whole rows
   const int kf = cc ? 0 : 1, kl = cc ? c->grid.ktot + 1 : c->grid.ktot;
   const long long nj = (long long)(kl - kf + 1) * 2 * c->gd.sy;
   k_jhalo<<<blocks_for(nj, 256), 256, 0, c->stream>>>(c->gd, kf, kl, o1, o2, o3);
@@ -446,8 +425,7 @@ void ibx_destroy(ibx_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  DevBuf* all[] = {&c->oboxes, &c->oubox, &c->oinfo, &c->oitems, &c->ocl, &c->oclcnt,
-                   &c->otouch, &c->oowner, &c->ofixc, &c->ofixo, &c->ofix3, &c->ometa,
+  DevBuf* all[] = {&c->oboxes, &c->oubox, &c->oitems, &c->osrc, &c->onsrc, &c->ocover, &c->ometa,
                    &c->tcount, &c->toff, &c->tpart, &c->tlist, &c->tmeta, &c->entries, &c->s_f1, &c->s_f2,
                    &c->s_f3,   &c->s_x,  &c->s_y,   &c->s_z,     &c->s_a1, &c->s_a2,
                    &c->s_a3,   &c->s_a4, &c->s_o1,  &c->s_o2,    &c->s_o3, &c->s_off};
@@ -940,12 +918,12 @@ int ibx_batch_profile(ibx_batch* b, int reps, double* us) {
   for (auto& e : ev) (void)hipEventDestroy(e);
   for (int i = 0; i < 6; ++i) us[i] = 1000.0 * acc[i] / reps;
   // work counts of the last run: tiles: non-empty tiles, tiles, candidate entries;
-  // objects: work items, cells recomputed by the fix-up, 0
+  // objects: work items, extra source objects (sum over items of sources - 1), 0
   if (kCfgs[b->cfg].obj) {
-    int m[5] = {0, 0, 0, 0, 0};
-    HIPCHK(c, hipMemcpyAsync(m, c->ometa.p, 5 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    int m[2] = {0, 0};
+    HIPCHK(c, hipMemcpyAsync(m, c->ometa.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    us[6] = (double)m[0], us[7] = (double)m[1], us[8] = (double)m[4];
+    us[6] = (double)m[0], us[7] = (double)m[1], us[8] = 0.0;
     return collect_errors(c);
   }
   int meta0 = 0;

@@ -841,16 +841,19 @@ __global__ void __launch_bounds__(SP_THREADS)
 }
 
 // ---------------------------------------------------------------------------------------
-// Spread by capsule objects.  An object is one capsule p in one image slot (loop index nb):
-// its candidates are the capsule's points in point order, which is the reference's K order
-// within the object (interp_spread.f90:825-878), so an object needs neither binning nor a
-// sort.  Different objects touch disjoint cells except where two capsules (or two images of
-// one capsule) come within a stencil width of each other; such cells are found on the fly
-// (a per-cell touch count) and recomputed by k_obj_fixup in the full K order.  Steps, on one
-// stream: zero the fields, the touch counts and the counters; k_obj_boxes (clamped window
-// bounds per object, work items of at most OB_NC cells); k_obj_spread (per item: the
-// candidates of the capsule whose window meets the item's box, in K order, chunks of 64 with
-// the ordered per-cell accumulation; writes the touched cells); k_obj_fixup; halos.
+// Spread by capsule objects.  An object is one capsule p in one image slot (loop index nb),
+// o = 9 p + nb; its candidates are the capsule's points in point order.  The reference adds
+// the terms of a cell in the loop order capsule -> nb -> point (interp_spread.f90:825-878),
+// i.e. in ascending (o, point) order, so an object needs neither binning nor a sort.
+// Work items are sub-boxes (at most OB_NC cells) of an object's box, the union of its
+// points' clamped windows.  An item takes as its sources every object whose box meets the
+// item's box, in ascending o, and accumulates their candidates in that order: every cell of
+// the item then receives all of its terms in the reference order, a cell lying in the boxes
+// of several items gets the same value from each, and each item writes its whole box (zeros
+// included).  Cells outside every item box are zeroed by the same persistent launch (zero
+// units interleaved with the items), which finds them in a bitmap of the covered cells.
+// Steps: k_obj_clear (bitmap, counters); k_obj_boxes (object boxes, items); k_obj_sources
+// (source lists, bitmap); k_obj_spread (items and zero units); k_jhalo.
 // ---------------------------------------------------------------------------------------
 constexpr int OB_BX = 32;          // item extent limit in x and y (cells)
 constexpr int OB_NZ = 32;          // item extent limit in z
@@ -858,20 +861,29 @@ constexpr int OB_NC = 2048;        // cells per item
 constexpr int OB_THREADS = 256;
 constexpr int OB_CPT = OB_NC / OB_THREADS;  // cells per thread
 constexpr int OB_PEND = 64 + OB_THREADS;    // pending candidate list
-constexpr int OB_INFO = 10;        // ints of per-object item-grid information
+constexpr int OB_NSRC = 32;        // source objects per item
+constexpr int OB_ZROWS = 8;        // field rows (i = 0..i1) per zero unit
+constexpr int OB_QBITS = 27;       // pending entry: point index | nb << OB_QBITS
+constexpr long long OB_MAXPTS = 1LL << OB_QBITS;
 
 struct ObjItem {
-  int p, nb, x0, y0, z0, nx, ny, nz, clo;  // clo: offset of the item's candidate list
+  int p, nb, x0, y0, z0, nx, ny, nz;
 };
 
-// counters in `meta`: [0] items, [1] fix-up pairs, [2] item queue, [4] cells touched by
-// three or more objects, [6..7] candidate-list space (64-bit)
+// counters in `meta`: [0] items, [1] extra sources (sum over items of sources - 1),
+// [2] item queue, [3] zero-unit queue
 // boxes: [nobj][6] clamped window bounds xa, xb, ya, yb, za, zb of object o = 9 p + nb
 // (xa > xb: no candidate); ubox: [ncap][6] their union per capsule
-// info: [nobj][OB_INFO] first item (-1: none), item counts cx, cy, cz, item extents nx, ny,
-// nz, box origin x0, y0, z0 (the items of an object are a cx x cy x cz grid, c fastest)
 
-// One wavefront per capsule.
+// the coverage bitmap (bit = linear cell index) and the counters
+__global__ void __launch_bounds__(256) k_obj_clear(long long nw, unsigned* __restrict__ cover,
+                                                   int* __restrict__ meta) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nw) cover[q] = 0u;
+  if (q < 8) meta[q] = 0;
+}
+
+// One wavefront per capsule: object boxes and the items that cut them.
 __global__ void __launch_bounds__(256) k_obj_boxes(GridD g, int klo, int khi, long long ncap,
                                                    const long long* __restrict__ cap_off,
                                                    const double* __restrict__ x,
@@ -879,9 +891,8 @@ __global__ void __launch_bounds__(256) k_obj_boxes(GridD g, int klo, int khi, lo
                                                    const double* __restrict__ z,
                                                    int* __restrict__ boxes,
                                                    int* __restrict__ ubox,
-                                                   int* __restrict__ info,
                                                    ObjItem* __restrict__ items, int icap,
-                                                   long long clcap, int* __restrict__ meta,
+                                                   int* __restrict__ meta,
                                                    int* __restrict__ err) {
   const int lane = threadIdx.x & 63;
   const long long p = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -927,7 +938,6 @@ __global__ void __launch_bounds__(256) k_obj_boxes(GridD g, int klo, int khi, lo
         bx[nb][e] = (e & 1) ? max(bx[nb][e], v) : min(bx[nb][e], v);
       }
   if (lane != 0) return;
-  const long long np_ = hi - lo;
   int u[6] = {1 << 30, -(1 << 30), 1 << 30, -(1 << 30), 1 << 30, -(1 << 30)};
   for (int nb = 0; nb < 9; ++nb) {
     const long long o = p * 9 + nb;
@@ -936,8 +946,6 @@ __global__ void __launch_bounds__(256) k_obj_boxes(GridD g, int klo, int khi, lo
       boxes[o * 6 + e] = has ? bx[nb][e] : (e & 1 ? 0 : 1);
       if (has) u[e] = (e & 1) ? max(u[e], bx[nb][e]) : min(u[e], bx[nb][e]);
     }
-    int* inf = info + o * OB_INFO;
-    inf[0] = -1;
     if (!has) continue;
     const int X = bx[nb][1] - bx[nb][0] + 1, Y = bx[nb][3] - bx[nb][2] + 1,
               Z = bx[nb][5] - bx[nb][4] + 1;
@@ -947,14 +955,16 @@ __global__ void __launch_bounds__(256) k_obj_boxes(GridD g, int klo, int khi, lo
     const int cz = (Z + nz - 1) / nz;
     const int cnt = cx * cy * cz;
     const int at = atomicAdd(&meta[0], cnt);
-    const long long need = (long long)cnt * np_;
-    const long long cl = (long long)atomicAdd((unsigned long long*)&meta[6], (unsigned long long)need);
-    if (at + cnt > icap || cl + need > clcap) {
+    if (at + cnt > icap) {
+      // the slots below icap that this object reserved become empty boxes (no sources, no
+      // cells), so the later kernels never read an unwritten item
+      ObjItem none;
+      none.p = (int)p, none.nb = nb, none.x0 = none.y0 = none.z0 = 1;
+      none.nx = none.ny = none.nz = 0;
+      for (int w = at; w < icap && w < at + cnt; ++w) items[w] = none;
       atomicOr(err, ERR_CAPACITY);
       continue;
     }
-    inf[0] = at, inf[1] = cx, inf[2] = cy, inf[3] = cz, inf[4] = nx, inf[5] = ny, inf[6] = nz;
-    inf[7] = bx[nb][0], inf[8] = bx[nb][2], inf[9] = bx[nb][4];
     int w = at;
     for (int a = 0; a < cx; ++a)
       for (int b = 0; b < cy; ++b)
@@ -965,7 +975,6 @@ __global__ void __launch_bounds__(256) k_obj_boxes(GridD g, int klo, int khi, lo
           it.nx = min(nx, bx[nb][1] - it.x0 + 1);
           it.ny = min(ny, bx[nb][3] - it.y0 + 1);
           it.nz = min(nz, bx[nb][5] - it.z0 + 1);
-          it.clo = (int)(cl + (long long)(w - at) * np_);
           items[w++] = it;
         }
   }
@@ -973,35 +982,103 @@ __global__ void __launch_bounds__(256) k_obj_boxes(GridD g, int klo, int khi, lo
   for (int e = 0; e < 6; ++e) ubox[p * 6 + e] = any ? u[e] : ((e & 1) ? 0 : 1);
 }
 
+// One wavefront per item (grid-stride): the objects whose boxes meet the item's box, in
+// ascending o, into src[item][0..nsrc[item]); the item's cells into the coverage bitmap.
+__global__ void __launch_bounds__(256) k_obj_sources(GridD g, long long ncap,
+                                                     const int* __restrict__ boxes,
+                                                     const int* __restrict__ ubox,
+                                                     const ObjItem* __restrict__ items,
+                                                     int icap, int* __restrict__ src,
+                                                     int* __restrict__ nsrc,
+                                                     unsigned* __restrict__ cover,
+                                                     int* __restrict__ meta,
+                                                     int* __restrict__ err) {
+  const int lane = threadIdx.x & 63;
+  const int nitems = min(meta[0], icap);
+  const int waves = gridDim.x * (blockDim.x >> 6);
+  for (int item = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); item < nitems;
+       item += waves) {
+    const ObjItem it = items[item];
+    const int xa = it.x0, xb = it.x0 + it.nx - 1, ya = it.y0, yb = it.y0 + it.ny - 1,
+              za = it.z0, zb = it.z0 + it.nz - 1;
+    int n = 0;
+    for (long long c0 = 0; c0 < ncap; c0 += 64) {
+      const long long p = c0 + lane;
+      bool hit = false;
+      if (p < ncap) {
+        const int* u = ubox + p * 6;
+        hit = u[0] <= xb && u[1] >= xa && u[2] <= yb && u[3] >= ya && u[4] <= zb && u[5] >= za;
+      }
+      unsigned long long bal = __ballot(hit);
+      while (bal) {  // capsules in ascending order, then nb ascending: ascending o
+        const long long pp = c0 + __builtin_ctzll(bal);
+        bal &= bal - 1;
+        bool h = false;
+        if (lane < 9) {
+          const int* b = boxes + (pp * 9 + lane) * 6;
+          h = b[0] <= b[1] && b[0] <= xb && b[1] >= xa && b[2] <= yb && b[3] >= ya &&
+              b[4] <= zb && b[5] >= za;
+        }
+        const unsigned long long hb = __ballot(h);
+        if (h) {
+          const int pos = n + __popcll(hb & __lanemask_lt());
+          if (pos < OB_NSRC) src[(long long)item * OB_NSRC + pos] = (int)(pp * 9 + lane);
+        }
+        n += __popcll(hb);
+      }
+    }
+    if (lane == 0) {
+      nsrc[item] = min(n, OB_NSRC);
+      if (n > OB_NSRC) atomicOr(err, ERR_CAPACITY);
+      if (n > 1) atomicAdd(&meta[1], n - 1);
+    }
+    // rows of the box: nx <= OB_BX = 32 cells, so at most two bitmap words each
+    const int rows = it.ny * it.nz;
+    for (int r = lane; r < rows; r += 64) {
+      const long long mm0 = (long long)it.x0 + (long long)(it.y0 + r % it.ny) * g.sy +
+                            (long long)(it.z0 + r / it.ny) * g.sz;
+      const long long mm1 = mm0 + it.nx - 1;
+      const unsigned b0 = (unsigned)(mm0 & 31), b1 = (unsigned)(mm1 & 31);
+      if ((mm0 >> 5) == (mm1 >> 5)) {
+        atomicOr(&cover[mm0 >> 5], (unsigned)(((1ull << (b1 - b0 + 1)) - 1ull) << b0));
+      } else {
+        atomicOr(&cover[mm0 >> 5], 0xffffffffu << b0);
+        atomicOr(&cover[mm1 >> 5], 0xffffffffu >> (31u - b1));
+      }
+    }
+  }
+}
+
 template <bool CC>
 struct ObjSpread {
   struct Smem {
     unsigned long long mx[OB_BX], my[OB_BX], mz[OB_NZ];  // candidates covering x / y / z
-    int pend[OB_PEND];
+    int pend[OB_PEND];          // point index | nb << OB_QBITS
     double4 fw[SP_CH][4];       // per candidate and x tap: f1*wx, f2*wx, f3*wx, area/dVeul
     double2 wyz[SP_CH][2][4];   // per candidate, y / z tap: (centred, staggered) weight
     int org[SP_CH];             // window origin relative to the item, x / y / z bytes, +8
     int rng[SP_CH][3];          // clamped window within the item, per direction: a | b << 8
+    int src[OB_NSRC];
     int wsum[OB_THREADS / 64];
-    int item;
+    int item, ns, zu;
     ObjItem it;
   };
 
-  // one chunk of m <= 64 candidates (point indices in sm.pend[0..m)), accumulated into acc
+  // one chunk of m <= 64 candidates (sm.pend[0..m)), accumulated into acc
   __device__ static void chunk(Smem& sm, int m, const GridD& g, int klo, int khi,
                                const double* __restrict__ x, const double* __restrict__ y,
                                const double* __restrict__ z, const double* __restrict__ f1,
                                const double* __restrict__ f2, const double* __restrict__ f3,
-                               const double* __restrict__ area, double (&acc)[OB_CPT][3],
-                               unsigned& touched) {
+                               const double* __restrict__ area, double (&acc)[OB_CPT][3]) {
     const int tid = threadIdx.x;
     const ObjItem& it = sm.it;
-    const int s = nb_slot(it.nb);
     const int ncell = it.nx * it.ny * it.nz;
     for (int c = tid >> 2; c < m; c += OB_THREADS / 4) {
       const int d = tid & 3;
       if (d == 3) continue;
-      const long long q = sm.pend[c];
+      const int e = sm.pend[c];
+      const long long q = e & (int)(OB_MAXPTS - 1);
+      const int s = nb_slot(e >> OB_QBITS);
       double cc_;
       int lo, hi, base_, ext;
       if (d == 0) {
@@ -1062,15 +1139,13 @@ struct ObjSpread {
       }
     }
     __syncthreads();
-    // ordered accumulation: bits ascending = K order within the object
+    // ordered accumulation: bits ascending = pending order = K order
 #pragma unroll
     for (int r = 0; r < OB_CPT; ++r) {
       const int cell = tid + r * OB_THREADS;
       if (cell >= ncell) continue;
       const int li = cell % it.nx, lj = (cell / it.nx) % it.ny, lk = cell / (it.nx * it.ny);
       unsigned long long mk = sm.mx[li] & sm.my[lj] & sm.mz[lk];
-      if (!mk) continue;
-      touched |= 1u << r;
       double a1 = acc[r][0], a2 = acc[r][1], a3 = acc[r][2];
       while (mk) {
         const int c = __builtin_ctzll(mk);
@@ -1094,22 +1169,94 @@ struct ObjSpread {
     }
     __syncthreads();
   }
+
+  // The item in sm.it with its sources in sm.src[0..sm.ns): candidates of each source in
+  // point order (order-preserving compaction of the points whose window meets the box),
+  // accumulated in chunks of SP_CH; then the whole box is written.
+  __device__ static void item(Smem& sm, const GridD& g, int klo, int khi,
+                              const long long* __restrict__ cap_off,
+                              const double* __restrict__ x, const double* __restrict__ y,
+                              const double* __restrict__ z, const double* __restrict__ f1,
+                              const double* __restrict__ f2, const double* __restrict__ f3,
+                              const double* __restrict__ area, double* __restrict__ o1,
+                              double* __restrict__ o2, double* __restrict__ o3) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const ObjItem it = sm.it;
+    double acc[OB_CPT][3];
+#pragma unroll
+    for (int r = 0; r < OB_CPT; ++r) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
+    int npend = 0;
+    const int ns = sm.ns;
+    for (int si = 0; si < ns; ++si) {
+      const int o = sm.src[si];
+      const int p = o / 9, nb = o - 9 * p;
+      const int s = nb_slot(nb), xs = slot_xs(s), ys = slot_ys(s);
+      const long long lo = cap_off[p], hi = cap_off[p + 1];
+      for (long long b0 = lo; b0 < hi; b0 += OB_THREADS) {
+        const long long q = b0 + tid;
+        bool cand = false;
+        if (q < hi) {
+          const double xq = x[q], yq = y[q], zq = z[q];
+          int w0, a, b;
+          if (isfinite(xq) && isfinite(yq) && isfinite(zq) &&
+              window(zq * g.dzi, klo, khi, w0, a, b) && a <= it.z0 + it.nz - 1 && b >= it.z0 &&
+              window(shifted_coord(xq, xs, g.lx, g.bl, g.dxi), 1, g.itot, w0, a, b) &&
+              a <= it.x0 + it.nx - 1 && b >= it.x0 &&
+              window(shifted_coord(yq, ys, g.ly, g.bf, g.dyi), 1, g.jtot, w0, a, b) &&
+              a <= it.y0 + it.ny - 1 && b >= it.y0)
+            cand = true;
+        }
+        const unsigned long long bal = __ballot(cand);
+        if (lane == 0) sm.wsum[wv] = __popcll(bal);
+        __syncthreads();
+        int before = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < OB_THREADS / 64; ++w) {
+          before += w < wv ? sm.wsum[w] : 0;
+          total += sm.wsum[w];
+        }
+        if (cand) sm.pend[npend + before + __popcll(bal & __lanemask_lt())] =
+            (int)q | (nb << OB_QBITS);
+        npend += total;
+        __syncthreads();
+        while (npend >= SP_CH) {
+          chunk(sm, SP_CH, g, klo, khi, x, y, z, f1, f2, f3, area, acc);
+          // move the rest (< OB_THREADS entries) to the front
+          const int rest = npend - SP_CH;
+          const int v = tid < rest ? sm.pend[SP_CH + tid] : 0;
+          __syncthreads();
+          if (tid < rest) sm.pend[tid] = v;
+          __syncthreads();
+          npend = rest;
+        }
+      }
+    }
+    if (npend > 0) chunk(sm, npend, g, klo, khi, x, y, z, f1, f2, f3, area, acc);
+    const int ncell = it.nx * it.ny * it.nz;
+#pragma unroll
+    for (int r = 0; r < OB_CPT; ++r) {
+      const int cell = tid + r * OB_THREADS;
+      if (cell >= ncell) continue;
+      const int li = cell % it.nx, lj = (cell / it.nx) % it.ny, lk = cell / (it.nx * it.ny);
+      const int i = it.x0 + li;
+      const long long mm =
+          (long long)i + (long long)(it.y0 + lj) * g.sy + (long long)(it.z0 + lk) * g.sz;
+      // the cell and, for i = itot or i = 1, its periodic x-halo duplicate (bound.f90:141-171:
+      // var(0) = var(imax), var(i1) = var(1)); the y halos are copied later by k_jhalo
+      o1[mm] = acc[r][0], o2[mm] = acc[r][1], o3[mm] = acc[r][2];
+      if (i == g.itot)
+        o1[mm - g.itot] = acc[r][0], o2[mm - g.itot] = acc[r][1], o3[mm - g.itot] = acc[r][2];
+      if (i == 1)
+        o1[mm + g.itot] = acc[r][0], o2[mm + g.itot] = acc[r][1], o3[mm + g.itot] = acc[r][2];
+    }
+  }
 };
 
-// Writes a cell and, for i = itot or i = 1, its periodic x-halo duplicate (bound.f90:141-171:
-// var(0) = var(imax), var(i1) = var(1)); the y halos are copied afterwards by k_jhalo.
-__device__ __forceinline__ void put_cell(const GridD& g, long long mm, int i, double v1,
-                                         double v2, double v3, double* __restrict__ o1,
-                                         double* __restrict__ o2, double* __restrict__ o3) {
-  o1[mm] = v1, o2[mm] = v2, o3[mm] = v3;
-  if (i == g.itot) o1[mm - g.itot] = v1, o2[mm - g.itot] = v2, o3[mm - g.itot] = v3;
-  if (i == 1) o1[mm + g.itot] = v1, o2[mm + g.itot] = v2, o3[mm + g.itot] = v3;
-}
-
-// Persistent: workgroups take items from the queue meta[2] until meta[0] is reached.  Each
-// touched cell is written and counted (bytes of `touch`); the first object to touch a cell
-// records itself in owner[], the second appends (cell, object) to the fix-up pairs, the
-// third appends the cell to the list of cells with three or more objects.
+// Persistent.  Each workgroup alternates: one item from the item queue meta[2], then up to
+// `per` zero units from meta[3] (OB_ZROWS whole rows i = 0..i1 each: the cells that no item
+// writes get 0, i.e. every cell whose bitmap bit is clear, halo cells i = 0 / i1 following
+// their sources itot / 1); once the items are gone, zero units only.  Items and zero units
+// write disjoint cells, so their order does not matter.
 template <bool CC>
 __global__ void __launch_bounds__(OB_THREADS)
     k_obj_spread(GridD g, int klo, int khi, const long long* __restrict__ cap_off,
@@ -1117,299 +1264,59 @@ __global__ void __launch_bounds__(OB_THREADS)
                  const double* __restrict__ z, const double* __restrict__ f1,
                  const double* __restrict__ f2, const double* __restrict__ f3,
                  const double* __restrict__ area, const ObjItem* __restrict__ items,
-                 int icap, int* __restrict__ meta, int* __restrict__ cl,
-                 int* __restrict__ clcnt, unsigned* __restrict__ touch,
-                 int* __restrict__ owner, int* __restrict__ fixc, int* __restrict__ fixo,
-                 int* __restrict__ fix3, int fcap, int* __restrict__ err,
-                 double* __restrict__ o1, double* __restrict__ o2, double* __restrict__ o3) {
+                 int icap, const int* __restrict__ src, const int* __restrict__ nsrc,
+                 const unsigned* __restrict__ cover, long long nrows,
+                 int* __restrict__ meta, double* __restrict__ o1, double* __restrict__ o2,
+                 double* __restrict__ o3) {
   using O = ObjSpread<CC>;
   __shared__ typename O::Smem sm;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x;
   // an overflowed item list was flagged by k_obj_boxes; never read past its capacity
   const int nitems = min(meta[0], icap);
-  for (;;) {
-    if (tid == 0) {
-      const int i = atomicAdd(&meta[2], 1);
-      sm.item = i;
-      if (i < nitems) sm.it = items[i];
-    }
-    __syncthreads();
-    const int item = sm.item;
-    if (item >= nitems) break;
-    const ObjItem it = sm.it;
-    const int s = nb_slot(it.nb), xs = slot_xs(s), ys = slot_ys(s);
-    const long long lo = cap_off[it.p], hi = cap_off[it.p + 1];
-    double acc[OB_CPT][3];
-#pragma unroll
-    for (int r = 0; r < OB_CPT; ++r) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
-    unsigned touched = 0;
-    int npend = 0, ncand = 0;
-    // candidates in point order: window meets the item box (order-preserving compaction);
-    // the list is also kept in cl[it.clo ...] for the fix-up
-    for (long long b0 = lo; b0 < hi; b0 += OB_THREADS) {
-      const long long q = b0 + tid;
-      bool cand = false;
-      if (q < hi) {
-        const double xq = x[q], yq = y[q], zq = z[q];
-        int w0, a, b;
-        if (isfinite(xq) && isfinite(yq) && isfinite(zq) &&
-            window(zq * g.dzi, klo, khi, w0, a, b) && a <= it.z0 + it.nz - 1 && b >= it.z0 &&
-            window(shifted_coord(xq, xs, g.lx, g.bl, g.dxi), 1, g.itot, w0, a, b) &&
-            a <= it.x0 + it.nx - 1 && b >= it.x0 &&
-            window(shifted_coord(yq, ys, g.ly, g.bf, g.dyi), 1, g.jtot, w0, a, b) &&
-            a <= it.y0 + it.ny - 1 && b >= it.y0)
-          cand = true;
+  const long long nzu = (nrows + OB_ZROWS - 1) / OB_ZROWS;
+  const int per = nitems > 0 ? (int)((nzu + nitems - 1) / nitems) : 0;
+  bool items_left = nitems > 0, zero_left = nzu > 0;
+  while (items_left || zero_left) {
+    if (items_left) {
+      if (tid == 0) {
+        const int i = atomicAdd(&meta[2], 1);
+        sm.item = i;
+        if (i < nitems) sm.it = items[i], sm.ns = nsrc[i];
       }
-      const unsigned long long bal = __ballot(cand);
-      if (lane == 0) sm.wsum[wv] = __popcll(bal);
       __syncthreads();
-      int before = 0, total = 0;
-#pragma unroll
-      for (int w = 0; w < OB_THREADS / 64; ++w) {
-        before += w < wv ? sm.wsum[w] : 0;
-        total += sm.wsum[w];
+      const int item = sm.item;
+      if (item < nitems) {
+        if (tid < sm.ns) sm.src[tid] = src[(long long)item * OB_NSRC + tid];
+        __syncthreads();
+        O::item(sm, g, klo, khi, cap_off, x, y, z, f1, f2, f3, area, o1, o2, o3);
+      } else {
+        items_left = false;
       }
-      if (cand) {
-        const int pos = before + __popcll(bal & __lanemask_lt());
-        sm.pend[npend + pos] = (int)q;
-        cl[it.clo + ncand + pos] = (int)q;
-      }
-      npend += total;
-      ncand += total;
       __syncthreads();
-      while (npend >= SP_CH) {
-        O::chunk(sm, SP_CH, g, klo, khi, x, y, z, f1, f2, f3, area, acc, touched);
-        // move the rest (< OB_THREADS entries) to the front
-        const int rest = npend - SP_CH;
-        const int v = tid < rest ? sm.pend[SP_CH + tid] : 0;
-        __syncthreads();
-        if (tid < rest) sm.pend[tid] = v;
-        __syncthreads();
-        npend = rest;
-      }
     }
-    if (npend > 0) O::chunk(sm, npend, g, klo, khi, x, y, z, f1, f2, f3, area, acc, touched);
-    if (tid == 0) clcnt[item] = ncand;
-    const int obj = it.p * 9 + it.nb;
-#pragma unroll
-    for (int r = 0; r < OB_CPT; ++r) {
-      if (!(touched & (1u << r))) continue;
-      const int cell = tid + r * OB_THREADS;
-      const int li = cell % it.nx, lj = (cell / it.nx) % it.ny, lk = cell / (it.nx * it.ny);
-      const int i = it.x0 + li;
-      const long long mm =
-          (long long)i + (long long)(it.y0 + lj) * g.sy + (long long)(it.z0 + lk) * g.sz;
-      put_cell(g, mm, i, acc[r][0], acc[r][1], acc[r][2], o1, o2, o3);
-      const unsigned sh = 8u * (unsigned)(mm & 3);
-      const unsigned prev = (atomicAdd(&touch[mm >> 2], 1u << sh) >> sh) & 0xffu;
-      if (prev == 0u) {
-        owner[mm] = obj;
-      } else if (prev == 1u) {
-        const int at = atomicAdd(&meta[1], 1);
-        if (at < fcap)
-          fixc[at] = (int)mm, fixo[at] = obj;
-        else
-          atomicOr(err, ERR_CAPACITY);
-      } else if (prev == 2u) {
-        const int at = atomicAdd(&meta[4], 1);
-        if (at < fcap)
-          fix3[at] = (int)mm;
-        else
-          atomicOr(err, ERR_CAPACITY);
-      }
-    }
-    __syncthreads();
-  }
-}
-
-// The term of candidate point q of object slot s at cell (i,j,k), or false if its clamped
-// window does not cover the cell.  Same operations as the accumulation above.
-template <bool CC>
-__device__ __forceinline__ bool obj_term(const GridD& g, int klo, int khi, int s, long long q,
-                                         int i, int j, int k, const double* __restrict__ x,
-                                         const double* __restrict__ y,
-                                         const double* __restrict__ z,
-                                         const double* __restrict__ f1,
-                                         const double* __restrict__ f2,
-                                         const double* __restrict__ f3,
-                                         const double* __restrict__ area, double& t1,
-                                         double& t2, double& t3) {
-  const double xq = x[q], yq = y[q], zq = z[q];
-  if (!(isfinite(xq) && isfinite(yq) && isfinite(zq))) return false;
-  const double cx = shifted_coord(xq, slot_xs(s), g.lx, g.bl, g.dxi);
-  const double cy = shifted_coord(yq, slot_ys(s), g.ly, g.bf, g.dyi);
-  const double cz = zq * g.dzi;
-  int w0, xa, xb, ya, yb, za, zb;
-  if (!(window(cx, 1, g.itot, w0, xa, xb) && i >= xa && i <= xb &&
-        window(cy, 1, g.jtot, w0, ya, yb) && j >= ya && j <= yb &&
-        window(cz, klo, khi, w0, za, zb) && k >= za && k <= zb))
-    return false;
-  Axis ax, ay, az;
-  axis_of(cx, 1, g.itot, ax);
-  axis_of(cy, 1, g.jtot, ay);
-  axis_of(cz, klo, khi, az);
-  const int ti = i - ax.w0, tj = j - ay.w0, tk = k - az.w0;
-  auto ns = [](const Axis& A, int t) {
-    const int u = t - A.na;
-    return u == 0 ? A.ns[0] : (u == 1 ? A.ns[1] : (u == 2 ? A.ns[2] : 0.0));
-  };
-  auto st = [](const Axis& A, int t) {
-    return t == 1 ? A.st[0] : (t == 2 ? A.st[1] : (t == 3 ? A.st[2] : 0.0));
-  };
-  const double adv = area[q] / g.dveul;
-  if (!CC) {
-    t1 = f1[q] * ns(ax, ti) * st(ay, tj) * st(az, tk) * adv;
-    t2 = f2[q] * st(ax, ti) * ns(ay, tj) * st(az, tk) * adv;
-    t3 = f3[q] * st(ax, ti) * st(ay, tj) * ns(az, tk) * adv;
-  } else {
-    t1 = f1[q] * st(ax, ti) * st(ay, tj) * st(az, tk) * adv;
-    t2 = f2[q] * st(ax, ti) * st(ay, tj) * st(az, tk) * adv;
-    t3 = f3[q] * st(ax, ti) * st(ay, tj) * st(az, tk) * adv;
-  }
-  return true;
-}
-
-// Cells touched by exactly two objects: the reference's sum over both, in K order (object
-// index 9 p + nb ascending, then point order), from the two items' saved candidate lists.
-// One wavefront per cell; every lane adds the covering points' terms in order (the same
-// sequential sum on all lanes; lane 0 stores it).
-template <bool CC>
-__global__ void __launch_bounds__(256)
-    k_obj_fixup2(GridD g, int klo, int khi, const int* __restrict__ info,
-                 const ObjItem* __restrict__ items, const int* __restrict__ cl,
-                 const int* __restrict__ clcnt, const unsigned* __restrict__ touch,
-                 const int* __restrict__ owner, const int* __restrict__ meta,
-                 const int* __restrict__ fixc, const int* __restrict__ fixo, int fcap,
-                 const double* __restrict__ x, const double* __restrict__ y,
-                 const double* __restrict__ z, const double* __restrict__ f1,
-                 const double* __restrict__ f2, const double* __restrict__ f3,
-                 const double* __restrict__ area, double* __restrict__ o1,
-                 double* __restrict__ o2, double* __restrict__ o3) {
-  const int nf = min(meta[1], fcap);
-  const int lane = threadIdx.x & 63, wpb = blockDim.x >> 6;
-  for (int f = blockIdx.x * wpb + (threadIdx.x >> 6); f < nf; f += gridDim.x * wpb) {
-    const long long mm = fixc[f];
-    const unsigned cnt = (touch[mm >> 2] >> (8u * (unsigned)(mm & 3))) & 0xffu;
-    if (cnt != 2u) continue;  // three or more objects: k_obj_fixup3
-    const int k = (int)(mm / g.sz), j = (int)((mm % g.sz) / g.sy), i = (int)(mm % g.sy);
-    const int oa = owner[mm], ob = fixo[f];
-    const int objs[2] = {min(oa, ob), max(oa, ob)};
-    double a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    for (int e = 0; e < 2; ++e) {
-      const int* inf = info + (long long)objs[e] * OB_INFO;
-      const int ia = (i - inf[7]) / inf[4], ib = (j - inf[8]) / inf[5], ic = (k - inf[9]) / inf[6];
-      const int item = inf[0] + (ia * inf[2] + ib) * inf[3] + ic;
-      const int s = nb_slot(objs[e] % 9);
-      const int* lst = cl + items[item].clo;
-      const int n = clcnt[item];
-      for (int c0 = 0; c0 < n; c0 += 64) {
-        double t1 = 0.0, t2 = 0.0, t3 = 0.0;
-        const bool cov = c0 + lane < n &&
-                         obj_term<CC>(g, klo, khi, s, lst[c0 + lane], i, j, k, x, y, z, f1, f2,
-                                      f3, area, t1, t2, t3);
-        unsigned long long c = __ballot(cov);
-        while (c) {
-          const int b = __builtin_ctzll(c);
-          c &= c - 1;
-          a1 = a1 + __shfl(t1, b);
-          a2 = a2 + __shfl(t2, b);
-          a3 = a3 + __shfl(t3, b);
+    if (zero_left) {
+      for (int t = 0; !items_left || t < per; ++t) {
+        if (tid == 0) sm.zu = atomicAdd(&meta[3], 1);
+        __syncthreads();
+        const long long u = sm.zu;
+        __syncthreads();
+        if (u >= nzu) {
+          zero_left = false;
+          break;
         }
-      }
-    }
-    if (lane == 0) put_cell(g, mm, i, a1, a2, a3, o1, o2, o3);
-  }
-}
-
-// Cells touched by three or more objects (rare): every capsule's box is tested, and the
-// points of each object covering the cell, in K order.
-template <bool CC>
-__global__ void __launch_bounds__(256)
-    k_obj_fixup3(GridD g, int klo, int khi, long long ncap, const long long* __restrict__ cap_off,
-                 const double* __restrict__ x, const double* __restrict__ y,
-                 const double* __restrict__ z, const double* __restrict__ f1,
-                 const double* __restrict__ f2, const double* __restrict__ f3,
-                 const double* __restrict__ area, const int* __restrict__ boxes,
-                 const int* __restrict__ ubox, const int* __restrict__ meta,
-                 const int* __restrict__ fix3, int fcap, double* __restrict__ o1,
-                 double* __restrict__ o2, double* __restrict__ o3) {
-  const int nf = min(meta[4], fcap);
-  const int lane = threadIdx.x & 63;
-  const int wpb = blockDim.x >> 6;
-  for (int f = blockIdx.x * wpb + (threadIdx.x >> 6); f < nf; f += gridDim.x * wpb) {
-    const long long mm = fix3[f];
-    const int k = (int)(mm / g.sz), j = (int)((mm % g.sz) / g.sy), i = (int)(mm % g.sy);
-    double a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    for (long long p0 = 0; p0 < ncap; p0 += 64) {
-      const long long pl = p0 + lane;
-      bool hit = false;
-      if (pl < ncap) {
-        const int* u = ubox + pl * 6;
-        hit = i >= u[0] && i <= u[1] && j >= u[2] && j <= u[3] && k >= u[4] && k <= u[5];
-      }
-      unsigned long long hits = __ballot(hit);
-      while (hits) {
-        const long long p = p0 + __builtin_ctzll(hits);
-        hits &= hits - 1;
-        for (int nb = 0; nb < 9; ++nb) {
-          const int* bx = boxes + (p * 9 + nb) * 6;
-          if (i < bx[0] || i > bx[1] || j < bx[2] || j > bx[3] || k < bx[4] || k > bx[5])
-            continue;
-          const int s = nb_slot(nb);
-          const long long lo = cap_off[p], hi = cap_off[p + 1];
-          for (long long q0 = lo; q0 < hi; q0 += 64) {
-            double t1 = 0.0, t2 = 0.0, t3 = 0.0;
-            const bool cov = q0 + lane < hi &&
-                             obj_term<CC>(g, klo, khi, s, q0 + lane, i, j, k, x, y, z, f1, f2,
-                                          f3, area, t1, t2, t3);
-            unsigned long long c = __ballot(cov);
-            while (c) {
-              const int b = __builtin_ctzll(c);
-              c &= c - 1;
-              a1 = a1 + __shfl(t1, b);
-              a2 = a2 + __shfl(t2, b);
-              a3 = a3 + __shfl(t3, b);
-            }
+        const long long r1 = min(nrows, (u + 1) * OB_ZROWS);
+        for (long long r = u * OB_ZROWS; r < r1; ++r) {
+          const long long base = r * g.sy;
+          for (int i = tid; i < (int)g.sy; i += OB_THREADS) {
+            const int ic = i == 0 ? g.itot : (i == g.itot + 1 ? 1 : i);
+            const long long mb = base + ic;
+            if (!((cover[mb >> 5] >> (mb & 31)) & 1u))
+              o1[base + i] = 0.0, o2[base + i] = 0.0, o3[base + i] = 0.0;
           }
         }
       }
     }
-    if (lane == 0) put_cell(g, mm, i, a1, a2, a3, o1, o2, o3);
   }
-}
-
-// x halos of every row j in [1,jtot], k in [kfirst,klast] (bound.f90:141-171):
-// var(0) = var(imax), var(i1) = var(1)
-__global__ void __launch_bounds__(256) k_xhalo(GridD g, int kfirst, int klast,
-                                               double* __restrict__ o1, double* __restrict__ o2,
-                                               double* __restrict__ o3) {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long rows = (long long)g.jtot * (klast - kfirst + 1);
-  if (q >= 2 * rows) return;
-  const long long r = q >> 1;
-  const long long row = (1 + r % g.jtot) * g.sy + (kfirst + r / g.jtot) * g.sz;
-  const long long dst = row + ((q & 1) ? g.itot + 1 : 0), src = row + ((q & 1) ? 1 : g.itot);
-  o1[dst] = o1[src], o2[dst] = o2[src], o3[dst] = o3[src];
-}
-
-// zero n doubles of each field (16-byte stores), n4 words of `touch`, and meta[0..7]
-__global__ void __launch_bounds__(256) k_obj_zero(long long n, double* __restrict__ o1,
-                                                  double* __restrict__ o2,
-                                                  double* __restrict__ o3, long long n4,
-                                                  unsigned* __restrict__ touch,
-                                                  int* __restrict__ meta) {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long e = 2 * q;
-  if (e + 1 < n) {
-    const double2 zz = make_double2(0.0, 0.0);
-    *reinterpret_cast<double2*>(o1 + e) = zz;
-    *reinterpret_cast<double2*>(o2 + e) = zz;
-    *reinterpret_cast<double2*>(o3 + e) = zz;
-  } else if (e < n) {
-    o1[e] = 0.0, o2[e] = 0.0, o3[e] = 0.0;
-  }
-  if (q < n4) touch[q] = 0u;
-  if (q < 8) meta[q] = 0;
 }
 
 // Wall halo planes k = 0 and k = k1 of lagr2eulr: zero (the scatter never reaches them and

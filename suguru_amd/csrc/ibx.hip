@@ -42,15 +42,18 @@ struct TileCfg {
   SpreadFn plain;
   SpreadFn cc;
   bool obj;       // spread by capsule objects instead of tiles
+  int minw;       // objects: minimum wavefronts per SIMD of the register allocation
 };
 
 #define IBX_CFG(TJ, TK, W, NAME) \
-  {TJ, TK, W, NAME, &k_spread<TJ, TK, false>, &k_spread<TJ, TK, true>, false}
-#define IBX_OBJ(W, NAME) {0, 0, W, NAME, nullptr, nullptr, true}
+  {TJ, TK, W, NAME, &k_spread<TJ, TK, false>, &k_spread<TJ, TK, true>, false, 0}
+#define IBX_OBJ(W, MINW, NAME) {0, 0, W, NAME, nullptr, nullptr, true, MINW}
 
 const TileCfg kCfgs[] = {
-    IBX_OBJ(4, "objects-w4"),
-    IBX_OBJ(8, "objects-w8"),
+    IBX_OBJ(4, 1, "objects-w4"),
+    IBX_OBJ(8, 1, "objects-w8"),
+    IBX_OBJ(4, 4, "objects-w4-o4"),
+    IBX_OBJ(8, 4, "objects-w8-o4"),
     IBX_CFG(8, 8, 8, "chunk16x8x8"),
     IBX_CFG(8, 4, 8, "chunk16x8x4"),
     IBX_CFG(4, 8, 8, "chunk16x4x8"),
@@ -78,7 +81,7 @@ struct ibx_ctx {
   // spread binning
   DevBuf tcount, toff, tpart, tlist, tmeta, entries;
   // spread by capsule objects
-  DevBuf oboxes, oubox, oitems, osrc, onsrc, ocover, ometa;
+  DevBuf oboxes, oubox, oitems, osrc, onsrc, ocover, ometa, onbm, ocnt, ooff, obinm;
   // staging for the host-array entry points
   DevBuf s_f1, s_f2, s_f3, s_x, s_y, s_z, s_a1, s_a2, s_a3, s_a4, s_o1, s_o2, s_o3, s_off;
   int cfg = 0;
@@ -185,9 +188,9 @@ int enqueue_scan(ibx_ctx* c, const unsigned* counts, long long n, unsigned* off)
   return IBX_OK;
 }
 
-// The spread by capsule objects on the context stream: bitmap and counters, object boxes
-// and items, sources and bitmap, the persistent item / zero-unit kernel, y halos (x halos
-// are written with the cells).  ev_tile / ev_tile_end (optional) bracket the persistent
+// The spread by capsule objects on the context stream: bitmap and counters, object boxes,
+// items and bitmap, capsule neighbours, sources, the persistent item / zero-unit kernel,
+// y halos (x halos are written with the cells).  ev_tile / ev_tile_end (optional) bracket the persistent
 // kernel; ev_split is recorded with ev_tile_end.
 int enqueue_spread_obj(ibx_ctx* c, const TileCfg& cf, bool cc, long long ncap,
                        const long long* d_off, long long npts, const double* x,
@@ -197,12 +200,20 @@ int enqueue_spread_obj(ibx_ctx* c, const TileCfg& cf, bool cc, long long ncap,
   if (npts >= OB_MAXPTS) return fail(c, IBX_E_ARG, "object spread: more than 2^27 points");
   const int klo = cc ? 0 : 1, khi = cc ? c->grid.ktot + 1 : c->grid.ktot;
   const long long ncells = c->gd.sz * (long long)(c->grid.ktot + 2);
-  const long long nrows = (long long)(c->grid.jtot + 2) * (c->grid.ktot + 2);
   const long long icap = std::max<long long>(1024, 64 * ncap + 1024);
   const long long nw = (ncells + 31) / 32;
   const long long nobj = std::max<long long>(1, 9 * ncap);
+  const int nwd = (int)((ncap + 63) / 64);
+  const long long nbins = (long long)((c->grid.itot + OB_BIN - 1) / OB_BIN) *
+                          ((c->grid.jtot + OB_BIN - 1) / OB_BIN) *
+                          ((c->grid.ktot + 2 + OB_BIN - 1) / OB_BIN);
+  const long long nbw = nbins * nwd;
   int rc;
   if ((rc = ensure(c, c->oboxes, (size_t)nobj * 6 * sizeof(int))) ||
+      (rc = ensure(c, c->onbm, (size_t)std::max<long long>(1, ncap * nwd) * 8)) ||
+      (rc = ensure(c, c->ocnt, (size_t)std::max<long long>(1, ncap) * sizeof(int))) ||
+      (rc = ensure(c, c->obinm, (size_t)std::max<long long>(1, nbw) * 8)) ||
+      (rc = ensure(c, c->ooff, (size_t)std::max<long long>(1, ncap) * sizeof(int))) ||
       (rc = ensure(c, c->oubox, (size_t)std::max<long long>(1, ncap) * 6 * sizeof(int))) ||
       (rc = ensure(c, c->oitems, (size_t)icap * sizeof(ObjItem))) ||
       (rc = ensure(c, c->osrc, (size_t)icap * OB_NSRC * sizeof(int))) ||
@@ -212,15 +223,21 @@ int enqueue_spread_obj(ibx_ctx* c, const TileCfg& cf, bool cc, long long ncap,
     return rc;
   int* meta = (int*)c->ometa.p;
   unsigned* cover = (unsigned*)c->ocover.p;
-  k_obj_clear<<<blocks_for(std::max<long long>(nw, 8), 256), 256, 0, c->stream>>>(nw, cover,
-                                                                                 meta);
+  k_obj_clear<<<blocks_for(std::max<long long>(std::max(nw, nbw), 8), 256), 256, 0,
+                c->stream>>>(nw, cover, nbw, (unsigned long long*)c->obinm.p, meta);
   if (npts > 0 && ncap > 0) {
-    k_obj_boxes<<<(unsigned)((ncap + 3) / 4), 256, 0, c->stream>>>(
-        c->gd, klo, khi, ncap, d_off, x, y, z, (int*)c->oboxes.p, (int*)c->oubox.p,
-        (ObjItem*)c->oitems.p, (int)icap, meta, c->d_err);
+    k_obj_boxes<<<(unsigned)ncap, 256, 0, c->stream>>>(
+        c->gd, klo, khi, d_off, x, y, z, (int*)c->oboxes.p, (int*)c->oubox.p, (int*)c->ocnt.p,
+        nwd, (unsigned long long*)c->obinm.p, c->d_err);
+    k_obj_scan<<<1, 1024, 0, c->stream>>>(ncap, (const int*)c->ocnt.p, (int*)c->ooff.p,
+                                          (int)icap, meta, c->d_err);
+    k_obj_items<<<(unsigned)ncap, 256, 0, c->stream>>>(
+        c->gd, ncap, nwd, (const int*)c->oboxes.p, (const unsigned long long*)c->obinm.p,
+        (const int*)c->ooff.p, (ObjItem*)c->oitems.p, (int)icap, cover,
+        (unsigned long long*)c->onbm.p);
     k_obj_sources<<<(unsigned)(c->num_cus * 4), 256, 0, c->stream>>>(
-        c->gd, ncap, (const int*)c->oboxes.p, (const int*)c->oubox.p,
-        (const ObjItem*)c->oitems.p, (int)icap, (int*)c->osrc.p, (int*)c->onsrc.p, cover, meta,
+        nwd, (const int*)c->oboxes.p, (const unsigned long long*)c->onbm.p,
+        (const ObjItem*)c->oitems.p, (int)icap, (int*)c->osrc.p, (int*)c->onsrc.p, meta,
         c->d_err);
   }
   if (ev_tile) HIPCHK(c, hipEventRecord(ev_tile, c->stream));
@@ -228,12 +245,19 @@ int enqueue_spread_obj(ibx_ctx* c, const TileCfg& cf, bool cc, long long ncap,
     const unsigned nblk = (unsigned)((long long)c->num_cus * cf.wg_per_cu);
 #define OBJ_ARGS                                                                             \
   c->gd, klo, khi, d_off, x, y, z, f1, f2, f3, area, (const ObjItem*)c->oitems.p, (int)icap, \
-      (const int*)c->osrc.p, (const int*)c->onsrc.p, (const unsigned*)cover, nrows, meta, o1, \
+      (const int*)c->osrc.p, (const int*)c->onsrc.p, (const unsigned*)cover, ncells, meta, o1, \
       o2, o3
-    if (cc)
-      k_obj_spread<true><<<nblk, OB_THREADS, 0, c->stream>>>(OBJ_ARGS);
-    else
-      k_obj_spread<false><<<nblk, OB_THREADS, 0, c->stream>>>(OBJ_ARGS);
+    if (cf.minw == 4) {
+      if (cc)
+        k_obj_spread<true, 4><<<nblk, OB_THREADS, 0, c->stream>>>(OBJ_ARGS);
+      else
+        k_obj_spread<false, 4><<<nblk, OB_THREADS, 0, c->stream>>>(OBJ_ARGS);
+    } else {
+      if (cc)
+        k_obj_spread<true, 1><<<nblk, OB_THREADS, 0, c->stream>>>(OBJ_ARGS);
+      else
+        k_obj_spread<false, 1><<<nblk, OB_THREADS, 0, c->stream>>>(OBJ_ARGS);
+    }
 #undef OBJ_ARGS
   }
   if (ev_tile_end) HIPCHK(c, hipEventRecord(ev_tile_end, c->stream));
@@ -425,7 +449,7 @@ void ibx_destroy(ibx_ctx* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  DevBuf* all[] = {&c->oboxes, &c->oubox, &c->oitems, &c->osrc, &c->onsrc, &c->ocover, &c->ometa,
+  DevBuf* all[] = {&c->oboxes, &c->oubox, &c->oitems, &c->osrc, &c->onsrc, &c->ocover, &c->ometa, &c->onbm, &c->ocnt, &c->ooff, &c->obinm,
                    &c->tcount, &c->toff, &c->tpart, &c->tlist, &c->tmeta, &c->entries, &c->s_f1, &c->s_f2,
                    &c->s_f3,   &c->s_x,  &c->s_y,   &c->s_z,     &c->s_a1, &c->s_a2,
                    &c->s_a3,   &c->s_a4, &c->s_o1,  &c->s_o2,    &c->s_o3, &c->s_off};
@@ -920,10 +944,17 @@ int ibx_batch_profile(ibx_batch* b, int reps, double* us) {
   // work counts of the last run: tiles: non-empty tiles, tiles, candidate entries;
   // objects: work items, extra source objects (sum over items of sources - 1), 0
   if (kCfgs[b->cfg].obj) {
-    int m[2] = {0, 0};
-    HIPCHK(c, hipMemcpyAsync(m, c->ometa.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    int m0 = 0;
+    HIPCHK(c, hipMemcpyAsync(&m0, c->ometa.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    us[6] = (double)m[0], us[7] = (double)m[1], us[8] = 0.0;
+    const int ni = (int)std::min<long long>(m0, c->onsrc.bytes / sizeof(int));
+    std::vector<int> ns((size_t)std::max(ni, 1));
+    if (ni > 0)
+      HIPCHK(c, hipMemcpy(ns.data(), c->onsrc.p, (size_t)ni * sizeof(int),
+                          hipMemcpyDeviceToHost));
+    long long extra = 0;
+    for (int i = 0; i < ni; ++i) extra += ns[i] - 1;
+    us[6] = (double)m0, us[7] = (double)extra, us[8] = 0.0;
     return collect_errors(c);
   }
   int meta0 = 0;

@@ -852,8 +852,9 @@ __global__ void __launch_bounds__(SP_THREADS)
 // of several items gets the same value from each, and each item writes its whole box (zeros
 // included).  Cells outside every item box are zeroed by the same persistent launch (zero
 // units interleaved with the items), which finds them in a bitmap of the covered cells.
-// Steps: k_obj_clear (bitmap, counters); k_obj_boxes (object boxes, items); k_obj_sources
-// (source lists, bitmap); k_obj_spread (items and zero units); k_jhalo.
+// Steps: k_obj_clear (bitmap, counters); k_obj_boxes (object boxes, items per capsule);
+// k_obj_scan (item offsets); k_obj_items (items, bitmap, neighbouring capsules);
+// k_obj_sources (source lists); k_obj_spread (items and zero units); k_jhalo.
 // ---------------------------------------------------------------------------------------
 constexpr int OB_BX = 32;          // item extent limit in x and y (cells)
 constexpr int OB_NZ = 32;          // item extent limit in z
@@ -862,7 +863,8 @@ constexpr int OB_THREADS = 256;
 constexpr int OB_CPT = OB_NC / OB_THREADS;  // cells per thread
 constexpr int OB_PEND = 64 + OB_THREADS;    // pending candidate list
 constexpr int OB_NSRC = 32;        // source objects per item
-constexpr int OB_ZROWS = 8;        // field rows (i = 0..i1) per zero unit
+constexpr int OB_ZIT = 8;          // cell pairs per thread in one zero unit
+constexpr long long OB_ZCELLS = 2LL * OB_THREADS * OB_ZIT;  // cells per zero unit
 constexpr int OB_QBITS = 27;       // pending entry: point index | nb << OB_QBITS
 constexpr long long OB_MAXPTS = 1LL << OB_QBITS;
 
@@ -870,39 +872,76 @@ struct ObjItem {
   int p, nb, x0, y0, z0, nx, ny, nz;
 };
 
-// counters in `meta`: [0] items, [1] extra sources (sum over items of sources - 1),
-// [2] item queue, [3] zero-unit queue
+// counters in `meta`: [0] items, [2] item queue, [3] zero-unit queue
 // boxes: [nobj][6] clamped window bounds xa, xb, ya, yb, za, zb of object o = 9 p + nb
 // (xa > xb: no candidate); ubox: [ncap][6] their union per capsule
 
-// the coverage bitmap (bit = linear cell index) and the counters
+// the coverage bitmap (bit = linear cell index), the bin masks and the counters
 __global__ void __launch_bounds__(256) k_obj_clear(long long nw, unsigned* __restrict__ cover,
+                                                   long long nbw,
+                                                   unsigned long long* __restrict__ binm,
                                                    int* __restrict__ meta) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < nw) cover[q] = 0u;
+  if (q < nbw) binm[q] = 0ull;
   if (q < 8) meta[q] = 0;
 }
 
-// One wavefront per capsule: object boxes and the items that cut them.
-__global__ void __launch_bounds__(256) k_obj_boxes(GridD g, int klo, int khi, long long ncap,
+// Item grid of one object box: cx x cy x cz sub-boxes of extent nx x ny x nz (the last ones
+// cut short), at most OB_BX x OB_BX x OB_NZ and OB_NC cells each.
+struct ObjGrid {
+  int cx, cy, cz, nx, ny, nz;
+  __device__ explicit ObjGrid(const int* b) {
+    const int X = b[1] - b[0] + 1, Y = b[3] - b[2] + 1, Z = b[5] - b[4] + 1;
+    cx = (X + OB_BX - 1) / OB_BX, cy = (Y + OB_BX - 1) / OB_BX;
+    nx = (X + cx - 1) / cx, ny = (Y + cy - 1) / cy;
+    nz = max(1, min(min(Z, OB_NZ), OB_NC / (nx * ny)));
+    cz = (Z + nz - 1) / nz;
+  }
+  __device__ int count() const { return cx * cy * cz; }
+};
+
+// Coarse bins of OB_BIN^3 cells over i in [1,itot], j in [1,jtot], k in [0,k1]: bin mask
+// binm[bin][0..nwd) has bit p set when a box of capsule p overlaps the bin.
+constexpr int OB_BIN = 32;
+struct ObjBins {
+  int nx, ny, nz;
+  __device__ explicit ObjBins(const GridD& g)
+      : nx((g.itot + OB_BIN - 1) / OB_BIN), ny((g.jtot + OB_BIN - 1) / OB_BIN),
+        nz((g.ktot + 2 + OB_BIN - 1) / OB_BIN) {}
+  // bin ranges of a (non-empty) box: bx0..bx1, by0..by1, bz0..bz1
+  __device__ void range(const int* b, int (&r)[6]) const {
+    r[0] = (b[0] - 1) / OB_BIN, r[1] = (b[1] - 1) / OB_BIN;
+    r[2] = (b[2] - 1) / OB_BIN, r[3] = (b[3] - 1) / OB_BIN;
+    r[4] = b[4] / OB_BIN, r[5] = b[5] / OB_BIN;
+  }
+  __device__ long long index(int bx, int by, int bz) const {
+    return (long long)bx + (long long)nx * (by + (long long)ny * bz);
+  }
+};
+
+// One workgroup per capsule: the clamped window boxes of its nine objects (boxes[o], o =
+// 9 p + nb), their union (ubox[p]), and its number of items (cnt[p]).
+__global__ void __launch_bounds__(256) k_obj_boxes(GridD g, int klo, int khi,
                                                    const long long* __restrict__ cap_off,
                                                    const double* __restrict__ x,
                                                    const double* __restrict__ y,
                                                    const double* __restrict__ z,
                                                    int* __restrict__ boxes,
                                                    int* __restrict__ ubox,
-                                                   ObjItem* __restrict__ items, int icap,
-                                                   int* __restrict__ meta,
+                                                   int* __restrict__ cnt, int nwd,
+                                                   unsigned long long* __restrict__ binm,
                                                    int* __restrict__ err) {
-  const int lane = threadIdx.x & 63;
-  const long long p = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (p >= ncap) return;  // whole wavefront
+  __shared__ int red[4][9][6];
+  __shared__ int ncnt[9];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const long long p = blockIdx.x;
   const long long lo = cap_off[p], hi = cap_off[p + 1];
   int bx[9][6];
 #pragma unroll
   for (int nb = 0; nb < 9; ++nb)
     bx[nb][0] = bx[nb][2] = bx[nb][4] = 1 << 30, bx[nb][1] = bx[nb][3] = bx[nb][5] = -(1 << 30);
-  for (long long q = lo + lane; q < hi; q += 64) {
+  for (long long q = lo + tid; q < hi; q += 256) {
     const double xq = x[q], yq = y[q], zq = z[q];
     if (!(isfinite(xq) && isfinite(yq) && isfinite(zq))) {
       atomicOr(err, ERR_NONFINITE);
@@ -929,68 +968,183 @@ __global__ void __launch_bounds__(256) k_obj_boxes(GridD g, int klo, int khi, lo
     }
   }
 #pragma unroll
-  for (int nb = 0; nb < 9; ++nb)
+  for (int nb = 0; nb < 9; ++nb) {
+    if (__ballot(bx[nb][0] <= bx[nb][1]) == 0ull) {  // no lane of the wavefront: skip
+      if (lane < 6) red[wv][nb][lane] = (lane & 1) ? -(1 << 30) : (1 << 30);
+      continue;
+    }
 #pragma unroll
-    for (int e = 0; e < 6; ++e)
+    for (int e = 0; e < 6; ++e) {
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
         const int v = __shfl_xor(bx[nb][e], o);
         bx[nb][e] = (e & 1) ? max(bx[nb][e], v) : min(bx[nb][e], v);
       }
-  if (lane != 0) return;
-  int u[6] = {1 << 30, -(1 << 30), 1 << 30, -(1 << 30), 1 << 30, -(1 << 30)};
-  for (int nb = 0; nb < 9; ++nb) {
-    const long long o = p * 9 + nb;
-    const bool has = bx[nb][0] <= bx[nb][1];
-    for (int e = 0; e < 6; ++e) {
-      boxes[o * 6 + e] = has ? bx[nb][e] : (e & 1 ? 0 : 1);
-      if (has) u[e] = (e & 1) ? max(u[e], bx[nb][e]) : min(u[e], bx[nb][e]);
+      if (lane == 0) red[wv][nb][e] = bx[nb][e];
     }
-    if (!has) continue;
-    const int X = bx[nb][1] - bx[nb][0] + 1, Y = bx[nb][3] - bx[nb][2] + 1,
-              Z = bx[nb][5] - bx[nb][4] + 1;
-    const int cx = (X + OB_BX - 1) / OB_BX, cy = (Y + OB_BX - 1) / OB_BX;
-    const int nx = (X + cx - 1) / cx, ny = (Y + cy - 1) / cy;
-    const int nz = max(1, min(min(Z, OB_NZ), OB_NC / (nx * ny)));
-    const int cz = (Z + nz - 1) / nz;
-    const int cnt = cx * cy * cz;
-    const int at = atomicAdd(&meta[0], cnt);
-    if (at + cnt > icap) {
-      // the slots below icap that this object reserved become empty boxes (no sources, no
-      // cells), so the later kernels never read an unwritten item
-      ObjItem none;
-      none.p = (int)p, none.nb = nb, none.x0 = none.y0 = none.z0 = 1;
-      none.nx = none.ny = none.nz = 0;
-      for (int w = at; w < icap && w < at + cnt; ++w) items[w] = none;
-      atomicOr(err, ERR_CAPACITY);
-      continue;
-    }
-    int w = at;
-    for (int a = 0; a < cx; ++a)
-      for (int b = 0; b < cy; ++b)
-        for (int c = 0; c < cz; ++c) {
-          ObjItem it;
-          it.p = (int)p, it.nb = nb;
-          it.x0 = bx[nb][0] + a * nx, it.y0 = bx[nb][2] + b * ny, it.z0 = bx[nb][4] + c * nz;
-          it.nx = min(nx, bx[nb][1] - it.x0 + 1);
-          it.ny = min(ny, bx[nb][3] - it.y0 + 1);
-          it.nz = min(nz, bx[nb][5] - it.z0 + 1);
-          items[w++] = it;
-        }
   }
-  const bool any = u[0] <= u[1];
-  for (int e = 0; e < 6; ++e) ubox[p * 6 + e] = any ? u[e] : ((e & 1) ? 0 : 1);
+  __syncthreads();
+  if (tid < 54) {
+    const int nb = tid / 6, e = tid % 6;
+    int v = red[0][nb][e];
+    for (int w = 1; w < 4; ++w) v = (e & 1) ? max(v, red[w][nb][e]) : min(v, red[w][nb][e]);
+    red[1][nb][e] = v;  // only this thread reads or writes element (nb, e) of red[1] here
+  }
+  __syncthreads();
+  const int(&B)[9][6] = red[1];
+  if (tid < 9) {
+    const int nb = tid;
+    const bool has = B[nb][0] <= B[nb][1];
+    for (int e = 0; e < 6; ++e) boxes[(p * 9 + nb) * 6 + e] = has ? B[nb][e] : (e & 1 ? 0 : 1);
+    ncnt[nb] = has ? ObjGrid(B[nb]).count() : 0;
+    if (has) {
+      const ObjBins bins(g);
+      int r[6];
+      bins.range(B[nb], r);
+      const unsigned long long bit = 1ull << (p & 63);
+      for (int bz = r[4]; bz <= r[5]; ++bz)
+        for (int by = r[2]; by <= r[3]; ++by)
+          for (int bx_ = r[0]; bx_ <= r[1]; ++bx_)
+            atomicOr(&binm[bins.index(bx_, by, bz) * nwd + (p >> 6)], bit);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    int u[6] = {1 << 30, -(1 << 30), 1 << 30, -(1 << 30), 1 << 30, -(1 << 30)};
+    int total = 0;
+    for (int nb = 0; nb < 9; ++nb) {
+      total += ncnt[nb];
+      if (B[nb][0] <= B[nb][1])
+        for (int e = 0; e < 6; ++e) u[e] = (e & 1) ? max(u[e], B[nb][e]) : min(u[e], B[nb][e]);
+    }
+    const bool any = u[0] <= u[1];
+    for (int e = 0; e < 6; ++e) ubox[p * 6 + e] = any ? u[e] : ((e & 1) ? 0 : 1);
+    cnt[p] = total;
+  }
+}
+
+// One workgroup of 1024 threads: off[p] = exclusive prefix sum of cnt over capsules, meta[0]
+// = the total number of items (flagged if above icap).
+__global__ void __launch_bounds__(1024) k_obj_scan(long long ncap, const int* __restrict__ cnt,
+                                                   int* __restrict__ off, int icap,
+                                                   int* __restrict__ meta,
+                                                   int* __restrict__ err) {
+  __shared__ int ws[16];
+  __shared__ int carry;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid == 0) carry = 0;
+  __syncthreads();
+  for (long long b = 0; b < ncap; b += 1024) {
+    const long long p = b + tid;
+    const int v = p < ncap ? cnt[p] : 0;
+    int s = v;  // inclusive scan within the wavefront
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(s, o);
+      if (lane >= o) s += t;
+    }
+    if (lane == 63) ws[wv] = s;
+    __syncthreads();
+    int before = carry;
+    for (int w = 0; w < wv; ++w) before += ws[w];
+    if (p < ncap) off[p] = before + s - v;
+    __syncthreads();
+    if (tid == 1023) carry = before + s;
+    __syncthreads();
+  }
+  if (tid == 0) {
+    meta[0] = carry;
+    if (carry > icap) atomicOr(err, ERR_CAPACITY);
+  }
+}
+
+// Sets the bits of cells [xa, xb] of row (j, k) in the coverage bitmap.
+__device__ __forceinline__ void obj_mark_row(const GridD& g, unsigned* __restrict__ cover,
+                                             int xa, int xb, int j, int k) {
+  const long long m0 = (long long)xa + (long long)j * g.sy + (long long)k * g.sz;
+  const long long m1 = m0 + (xb - xa);
+  for (long long w = m0 >> 5; w <= (m1 >> 5); ++w) {
+    const long long a = max(m0, w << 5), b = min(m1, (w << 5) + 31);
+    const unsigned lo = (unsigned)(a & 31), n = (unsigned)(b - a + 1);
+    atomicOr(&cover[w], (unsigned)(((1ull << n) - 1ull) << lo));
+  }
+}
+
+__device__ __forceinline__ bool boxes_meet(const int* a, const int* b) {
+  return a[0] <= a[1] && b[0] <= b[1] && a[0] <= b[1] && a[1] >= b[0] && a[2] <= b[3] &&
+         a[3] >= b[2] && a[4] <= b[5] && a[5] >= b[4];
+}
+
+// One workgroup per capsule p: its items (slots off[p] ... below icap; the items of an
+// object partition its box, c fastest), its boxes' cells in the coverage bitmap, and its
+// neighbour mask: bit q of nbm[p][0..nwd) is set iff a box of capsule q meets a box of p
+// (p itself included when it has a box); the candidates q are the capsules registered in
+// the bins that p's boxes overlap.
+__global__ void __launch_bounds__(256) k_obj_items(GridD g, long long ncap, int nwd,
+                                                   const int* __restrict__ boxes,
+                                                   const unsigned long long* __restrict__ binm,
+                                                   const int* __restrict__ off,
+                                                   ObjItem* __restrict__ items, int icap,
+                                                   unsigned* __restrict__ cover,
+                                                   unsigned long long* __restrict__ nbm) {
+  __shared__ int B[9][6];
+  const int tid = threadIdx.x;
+  const long long p = blockIdx.x;
+  if (tid < 54) B[tid / 6][tid % 6] = boxes[p * 54 + tid];
+  __syncthreads();
+  int at = off[p];
+  for (int nb = 0; nb < 9; ++nb) {
+    if (B[nb][0] > B[nb][1]) continue;  // uniform
+    const ObjGrid og(B[nb]);
+    for (int t = tid; t < og.count(); t += 256) {
+      if (at + t >= icap) break;  // flagged by k_obj_scan
+      const int a = t / (og.cy * og.cz), b = (t / og.cz) % og.cy, c = t % og.cz;
+      ObjItem it;
+      it.p = (int)p, it.nb = nb;
+      it.x0 = B[nb][0] + a * og.nx, it.y0 = B[nb][2] + b * og.ny, it.z0 = B[nb][4] + c * og.nz;
+      it.nx = min(og.nx, B[nb][1] - it.x0 + 1);
+      it.ny = min(og.ny, B[nb][3] - it.y0 + 1);
+      it.nz = min(og.nz, B[nb][5] - it.z0 + 1);
+      items[at + t] = it;
+    }
+    at += og.count();
+    const int Y = B[nb][3] - B[nb][2] + 1, Z = B[nb][5] - B[nb][4] + 1;
+    for (int r = tid; r < Y * Z; r += 256)
+      obj_mark_row(g, cover, B[nb][0], B[nb][1], B[nb][2] + r % Y, B[nb][4] + r / Y);
+  }
+  const ObjBins bins(g);
+  for (int w = tid; w < nwd; w += 256) {
+    unsigned long long cand = 0ull;
+    for (int nb = 0; nb < 9; ++nb) {
+      if (B[nb][0] > B[nb][1]) continue;
+      int r[6];
+      bins.range(B[nb], r);
+      for (int bz = r[4]; bz <= r[5]; ++bz)
+        for (int by = r[2]; by <= r[3]; ++by)
+          for (int bx = r[0]; bx <= r[1]; ++bx) cand |= binm[bins.index(bx, by, bz) * nwd + w];
+    }
+    unsigned long long res = 0ull;
+    while (cand) {
+      const int l = __builtin_ctzll(cand);
+      cand &= cand - 1;
+      const int* bq = boxes + ((long long)w * 64 + l) * 54;
+      bool hit = false;
+      for (int n2 = 0; n2 < 9 && !hit; ++n2)
+        for (int n1 = 0; n1 < 9 && !hit; ++n1) hit = boxes_meet(B[n1], bq + n2 * 6);
+      if (hit) res |= 1ull << l;
+    }
+    nbm[p * nwd + w] = res;
+  }
 }
 
 // One wavefront per item (grid-stride): the objects whose boxes meet the item's box, in
-// ascending o, into src[item][0..nsrc[item]); the item's cells into the coverage bitmap.
-__global__ void __launch_bounds__(256) k_obj_sources(GridD g, long long ncap,
-                                                     const int* __restrict__ boxes,
-                                                     const int* __restrict__ ubox,
+// ascending o, into src[item][0..nsrc[item]).  Candidates: the neighbours of the item's
+// capsule (the item's box lies in its capsule's union box).
+__global__ void __launch_bounds__(256) k_obj_sources(int nwd, const int* __restrict__ boxes,
+                                                     const unsigned long long* __restrict__ nbm,
                                                      const ObjItem* __restrict__ items,
                                                      int icap, int* __restrict__ src,
                                                      int* __restrict__ nsrc,
-                                                     unsigned* __restrict__ cover,
                                                      int* __restrict__ meta,
                                                      int* __restrict__ err) {
   const int lane = threadIdx.x & 63;
@@ -1001,50 +1155,38 @@ __global__ void __launch_bounds__(256) k_obj_sources(GridD g, long long ncap,
     const ObjItem it = items[item];
     const int xa = it.x0, xb = it.x0 + it.nx - 1, ya = it.y0, yb = it.y0 + it.ny - 1,
               za = it.z0, zb = it.z0 + it.nz - 1;
+    const unsigned long long* row = nbm + (long long)it.p * nwd;
     int n = 0;
-    for (long long c0 = 0; c0 < ncap; c0 += 64) {
-      const long long p = c0 + lane;
-      bool hit = false;
-      if (p < ncap) {
-        const int* u = ubox + p * 6;
-        hit = u[0] <= xb && u[1] >= xa && u[2] <= yb && u[3] >= ya && u[4] <= zb && u[5] >= za;
-      }
-      unsigned long long bal = __ballot(hit);
-      while (bal) {  // capsules in ascending order, then nb ascending: ascending o
-        const long long pp = c0 + __builtin_ctzll(bal);
-        bal &= bal - 1;
-        bool h = false;
-        if (lane < 9) {
-          const int* b = boxes + (pp * 9 + lane) * 6;
-          h = b[0] <= b[1] && b[0] <= xb && b[1] >= xa && b[2] <= yb && b[3] >= ya &&
-              b[4] <= zb && b[5] >= za;
+    for (int w0 = 0; w0 < nwd; w0 += 64) {
+      const unsigned long long word = w0 + lane < nwd ? row[w0 + lane] : 0ull;
+      unsigned long long nzw = __ballot(word != 0ull);
+      while (nzw) {  // words ascending, then bits ascending: capsules ascending
+        const int l = __builtin_ctzll(nzw);
+        nzw &= nzw - 1;
+        const unsigned lo32 = (unsigned)__shfl((int)(unsigned)word, l),
+                       hi32 = (unsigned)__shfl((int)(unsigned)(word >> 32), l);
+        unsigned long long bits = ((unsigned long long)hi32 << 32) | lo32;
+        while (bits) {
+          const long long q = (long long)(w0 + l) * 64 + __builtin_ctzll(bits);
+          bits &= bits - 1;
+          bool h = false;
+          if (lane < 9) {
+            const int* b = boxes + (q * 9 + lane) * 6;
+            h = b[0] <= b[1] && b[0] <= xb && b[1] >= xa && b[2] <= yb && b[3] >= ya &&
+                b[4] <= zb && b[5] >= za;
+          }
+          const unsigned long long hb = __ballot(h);
+          if (h) {
+            const int pos = n + __popcll(hb & __lanemask_lt());
+            if (pos < OB_NSRC) src[(long long)item * OB_NSRC + pos] = (int)(q * 9 + lane);
+          }
+          n += __popcll(hb);
         }
-        const unsigned long long hb = __ballot(h);
-        if (h) {
-          const int pos = n + __popcll(hb & __lanemask_lt());
-          if (pos < OB_NSRC) src[(long long)item * OB_NSRC + pos] = (int)(pp * 9 + lane);
-        }
-        n += __popcll(hb);
       }
     }
     if (lane == 0) {
       nsrc[item] = min(n, OB_NSRC);
       if (n > OB_NSRC) atomicOr(err, ERR_CAPACITY);
-      if (n > 1) atomicAdd(&meta[1], n - 1);
-    }
-    // rows of the box: nx <= OB_BX = 32 cells, so at most two bitmap words each
-    const int rows = it.ny * it.nz;
-    for (int r = lane; r < rows; r += 64) {
-      const long long mm0 = (long long)it.x0 + (long long)(it.y0 + r % it.ny) * g.sy +
-                            (long long)(it.z0 + r / it.ny) * g.sz;
-      const long long mm1 = mm0 + it.nx - 1;
-      const unsigned b0 = (unsigned)(mm0 & 31), b1 = (unsigned)(mm1 & 31);
-      if ((mm0 >> 5) == (mm1 >> 5)) {
-        atomicOr(&cover[mm0 >> 5], (unsigned)(((1ull << (b1 - b0 + 1)) - 1ull) << b0));
-      } else {
-        atomicOr(&cover[mm0 >> 5], 0xffffffffu << b0);
-        atomicOr(&cover[mm1 >> 5], 0xffffffffu >> (31u - b1));
-      }
     }
   }
 }
@@ -1252,20 +1394,63 @@ struct ObjSpread {
   }
 };
 
+__device__ __forceinline__ bool obj_covered(const unsigned* __restrict__ cover, long long m) {
+  return (cover[m >> 5] >> (m & 31)) & 1u;
+}
+
+// Zero unit u: cells [u OB_ZCELLS, (u+1) OB_ZCELLS) of the linear index, as pairs
+// (2 tid + 2 OB_THREADS e, +1): every cell that no item writes gets 0, i.e. every cell whose
+// bitmap bit is clear -- for the halo cells i = 0 and i1, the bit of their sources itot and 1
+// (the items write those duplicates).  All bitmap words are read before the stores.
+__device__ __forceinline__ void obj_zero_unit(const GridD& g, long long ncells, long long u,
+                                              const unsigned* __restrict__ cover,
+                                              double* __restrict__ o1,
+                                              double* __restrict__ o2,
+                                              double* __restrict__ o3) {
+  const long long m0 = u * OB_ZCELLS + 2 * threadIdx.x;
+  const int sy = (int)g.sy, step = (int)((2LL * OB_THREADS) % g.sy);
+  int i = (int)(m0 % g.sy);
+  bool za[OB_ZIT], zb[OB_ZIT];
+#pragma unroll
+  for (int e = 0; e < OB_ZIT; ++e) {
+    const long long m = m0 + 2LL * OB_THREADS * e;
+    const int i1 = i + 1 == sy ? 0 : i + 1;
+    const long long sa = i == 0 ? m + g.itot : (i == sy - 1 ? m - g.itot : m);
+    const long long sb = i1 == 0 ? m + 1 + g.itot : (i1 == sy - 1 ? m + 1 - g.itot : m + 1);
+    za[e] = m < ncells && !obj_covered(cover, sa);
+    zb[e] = m + 1 < ncells && !obj_covered(cover, sb);
+    i += step;
+    if (i >= sy) i -= sy;
+  }
+#pragma unroll
+  for (int e = 0; e < OB_ZIT; ++e) {
+    const long long m = m0 + 2LL * OB_THREADS * e;
+    if (za[e] && zb[e]) {
+      const double2 zz = make_double2(0.0, 0.0);
+      *reinterpret_cast<double2*>(o1 + m) = zz;
+      *reinterpret_cast<double2*>(o2 + m) = zz;
+      *reinterpret_cast<double2*>(o3 + m) = zz;
+    } else {
+      if (za[e]) o1[m] = 0.0, o2[m] = 0.0, o3[m] = 0.0;
+      if (zb[e]) o1[m + 1] = 0.0, o2[m + 1] = 0.0, o3[m + 1] = 0.0;
+    }
+  }
+}
+
 // Persistent.  Each workgroup alternates: one item from the item queue meta[2], then up to
-// `per` zero units from meta[3] (OB_ZROWS whole rows i = 0..i1 each: the cells that no item
-// writes get 0, i.e. every cell whose bitmap bit is clear, halo cells i = 0 / i1 following
-// their sources itot / 1); once the items are gone, zero units only.  Items and zero units
-// write disjoint cells, so their order does not matter.
-template <bool CC>
-__global__ void __launch_bounds__(OB_THREADS)
+// `per` of its own zero units (units blockIdx.x, blockIdx.x + gridDim.x, ...: a static
+// share, no counter to contend for); once the items are gone, the rest of its units.  Items
+// and zero units write disjoint cells, so their order does not matter.  MINW: the minimum wavefronts
+// per SIMD the register allocation must allow (1: unconstrained).
+template <bool CC, int MINW>
+__global__ void __launch_bounds__(OB_THREADS, MINW)
     k_obj_spread(GridD g, int klo, int khi, const long long* __restrict__ cap_off,
                  const double* __restrict__ x, const double* __restrict__ y,
                  const double* __restrict__ z, const double* __restrict__ f1,
                  const double* __restrict__ f2, const double* __restrict__ f3,
                  const double* __restrict__ area, const ObjItem* __restrict__ items,
                  int icap, const int* __restrict__ src, const int* __restrict__ nsrc,
-                 const unsigned* __restrict__ cover, long long nrows,
+                 const unsigned* __restrict__ cover, long long ncells,
                  int* __restrict__ meta, double* __restrict__ o1, double* __restrict__ o2,
                  double* __restrict__ o3) {
   using O = ObjSpread<CC>;
@@ -1273,9 +1458,11 @@ __global__ void __launch_bounds__(OB_THREADS)
   const int tid = threadIdx.x;
   // an overflowed item list was flagged by k_obj_boxes; never read past its capacity
   const int nitems = min(meta[0], icap);
-  const long long nzu = (nrows + OB_ZROWS - 1) / OB_ZROWS;
-  const int per = nitems > 0 ? (int)((nzu + nitems - 1) / nitems) : 0;
-  bool items_left = nitems > 0, zero_left = nzu > 0;
+  const long long nzu = (ncells + OB_ZCELLS - 1) / OB_ZCELLS;
+  const long long mine = nzu > blockIdx.x ? (nzu - blockIdx.x + gridDim.x - 1) / gridDim.x : 0;
+  const int per = nitems > 0 ? (int)((mine * gridDim.x + nitems - 1) / nitems) : 0;
+  long long zu = blockIdx.x;
+  bool items_left = nitems > 0, zero_left = mine > 0;
   while (items_left || zero_left) {
     if (items_left) {
       if (tid == 0) {
@@ -1296,24 +1483,12 @@ __global__ void __launch_bounds__(OB_THREADS)
     }
     if (zero_left) {
       for (int t = 0; !items_left || t < per; ++t) {
-        if (tid == 0) sm.zu = atomicAdd(&meta[3], 1);
-        __syncthreads();
-        const long long u = sm.zu;
-        __syncthreads();
-        if (u >= nzu) {
+        if (zu >= nzu) {
           zero_left = false;
           break;
         }
-        const long long r1 = min(nrows, (u + 1) * OB_ZROWS);
-        for (long long r = u * OB_ZROWS; r < r1; ++r) {
-          const long long base = r * g.sy;
-          for (int i = tid; i < (int)g.sy; i += OB_THREADS) {
-            const int ic = i == 0 ? g.itot : (i == g.itot + 1 ? 1 : i);
-            const long long mb = base + ic;
-            if (!((cover[mb >> 5] >> (mb & 31)) & 1u))
-              o1[base + i] = 0.0, o2[base + i] = 0.0, o3[base + i] = 0.0;
-          }
-        }
+        obj_zero_unit(g, ncells, zu, cover, o1, o2, o3);
+        zu += gridDim.x;
       }
     }
   }

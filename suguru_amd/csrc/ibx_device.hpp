@@ -857,10 +857,9 @@ __global__ void __launch_bounds__(SP_THREADS)
 // k_obj_sources (source lists); k_obj_spread (items and zero units); k_jhalo.
 // ---------------------------------------------------------------------------------------
 constexpr int OB_BX = 32;          // item extent limit in x and y (cells)
-constexpr int OB_NZ = 32;          // item extent limit in z
-constexpr int OB_NC = 2048;        // cells per item
-constexpr int OB_THREADS = 256;
-constexpr int OB_CPT = OB_NC / OB_THREADS;  // cells per thread
+constexpr int OB_THREADS = 256;    // also the limit of an item's columns nx * ny
+constexpr int OB_CPT = 8;          // item extent limit in z: one column of cells per thread
+constexpr int OB_NC = OB_THREADS * OB_CPT;  // cells per item
 constexpr int OB_PEND = 64 + OB_THREADS;    // pending candidate list
 constexpr int OB_NSRC = 32;        // source objects per item
 constexpr int OB_ZIT = 8;          // cell pairs per thread in one zero unit
@@ -888,17 +887,29 @@ __global__ void __launch_bounds__(256) k_obj_clear(long long nw, unsigned* __res
 }
 
 // Item grid of one object box: cx x cy x cz sub-boxes of extent nx x ny x nz (the last ones
-// cut short), at most OB_BX x OB_BX x OB_NZ and OB_NC cells each.
+// cut short) with nx, ny <= OB_BX, nx * ny <= OB_THREADS columns and nz <= OB_CPT.  Of the
+// few x splits tried (the fewest pieces and up to two more), the one of least estimated
+// work: per item, the cells its candidates' windows can reach, (nx+3)(ny+3)(nz+3), plus a
+// fixed cost for the per-item candidate scan.
 struct ObjGrid {
   int cx, cy, cz, nx, ny, nz;
   __device__ explicit ObjGrid(const int* b) {
     const int X = b[1] - b[0] + 1, Y = b[3] - b[2] + 1, Z = b[5] - b[4] + 1;
-    cx = (X + OB_BX - 1) / OB_BX, cy = (Y + OB_BX - 1) / OB_BX;
-    nx = (X + cx - 1) / cx, ny = (Y + cy - 1) / cy;
-    nz = max(1, min(min(Z, OB_NZ), OB_NC / (nx * ny)));
+    nz = min(Z, OB_CPT);
     cz = (Z + nz - 1) / nz;
+    long long best = -1;
+    const int a0 = (X + OB_BX - 1) / OB_BX;
+    for (int ax = a0; ax <= min(X, a0 + 2); ++ax) {
+      const int ex = (X + ax - 1) / ax;
+      const int ay = (Y + min(OB_BX, OB_THREADS / ex) - 1) / min(OB_BX, OB_THREADS / ex);
+      const int ey = (Y + ay - 1) / ay;
+      const long long cost =
+          (long long)ax * ay * cz * ((long long)(ex + 3) * (ey + 3) * (nz + 3) + kItemCost);
+      if (best < 0 || cost < best) best = cost, cx = ax, cy = ay, nx = ex, ny = ey;
+    }
   }
   __device__ int count() const { return cx * cy * cz; }
+  static constexpr long long kItemCost = 1000;
 };
 
 // Coarse bins of OB_BIN^3 cells over i in [1,itot], j in [1,jtot], k in [0,k1]: bin mask
@@ -1194,7 +1205,7 @@ __global__ void __launch_bounds__(256) k_obj_sources(int nwd, const int* __restr
 template <bool CC>
 struct ObjSpread {
   struct Smem {
-    unsigned long long mx[OB_BX], my[OB_BX], mz[OB_NZ];  // candidates covering x / y / z
+    unsigned long long mx[OB_BX], my[OB_BX];  // candidates whose window covers x / y
     int pend[OB_PEND];          // point index | nb << OB_QBITS
     double4 fw[SP_CH][4];       // per candidate and x tap: f1*wx, f2*wx, f3*wx, area/dVeul
     double2 wyz[SP_CH][2][4];   // per candidate, y / z tap: (centred, staggered) weight
@@ -1214,7 +1225,6 @@ struct ObjSpread {
                                const double* __restrict__ area, double (&acc)[OB_CPT][3]) {
     const int tid = threadIdx.x;
     const ObjItem& it = sm.it;
-    const int ncell = it.nx * it.ny * it.nz;
     for (int c = tid >> 2; c < m; c += OB_THREADS / 4) {
       const int d = tid & 3;
       if (d == 3) continue;
@@ -1264,14 +1274,15 @@ struct ObjSpread {
       reinterpret_cast<unsigned char*>(&sm.org[c])[d] = (unsigned char)(w0 + 8);
     }
     __syncthreads();
-    // separable masks: candidate c touches cell (li, lj, lk) iff its clamped window covers li,
-    // lj and lk, so the cell's mask is mx[li] & my[lj] & mz[lk]; wavefront w < 3 builds the
-    // masks of direction w with one ballot per position (lane = candidate)
+    // separable masks: candidate c touches column (li, lj) iff its clamped window covers li and
+    // lj, so the column's mask is mx[li] & my[lj]; wavefront w < 2 builds the masks of
+    // direction w with one ballot per position (lane = candidate).  Every candidate's window
+    // meets the item's z range.
     {
       const int wv = tid >> 6, lane = tid & 63;
-      if (wv < 3) {
-        const int ext = wv == 0 ? it.nx : (wv == 1 ? it.ny : it.nz);
-        unsigned long long* dst = wv == 0 ? sm.mx : (wv == 1 ? sm.my : sm.mz);
+      if (wv < 2) {
+        const int ext = wv == 0 ? it.nx : it.ny;
+        unsigned long long* dst = wv == 0 ? sm.mx : sm.my;
         const int r = lane < m ? sm.rng[lane][wv] : 0x00ff;  // a = 255 > b: covers nothing
         const int a = r & 0xff, b = r >> 8;
         for (int l = 0; l < ext; ++l) {
@@ -1281,33 +1292,32 @@ struct ObjSpread {
       }
     }
     __syncthreads();
-    // ordered accumulation: bits ascending = pending order = K order
-#pragma unroll
-    for (int r = 0; r < OB_CPT; ++r) {
-      const int cell = tid + r * OB_THREADS;
-      if (cell >= ncell) continue;
-      const int li = cell % it.nx, lj = (cell / it.nx) % it.ny, lk = cell / (it.nx * it.ny);
-      unsigned long long mk = sm.mx[li] & sm.my[lj] & sm.mz[lk];
-      double a1 = acc[r][0], a2 = acc[r][1], a3 = acc[r][2];
+    // ordered accumulation: thread = column (li, lj), its cells lk = 0..nz-1 in acc[lk]; bits
+    // ascending = pending order = K order.  The term ((f wx) wy) wz (area/dVeul) keeps the
+    // reference's order of products; its first two factors are formed once per candidate and
+    // column, then multiplied by the z tap of each cell in the clamped z window.
+    if (tid < it.nx * it.ny) {
+      const int li = tid % it.nx, lj = tid / it.nx;
+      unsigned long long mk = sm.mx[li] & sm.my[lj];
       while (mk) {
         const int c = __builtin_ctzll(mk);
         mk &= mk - 1;
         const unsigned o = (unsigned)sm.org[c];
-        const int a = li + 8 - (int)(o & 0xff), b = lj + 8 - (int)((o >> 8) & 0xff),
-                  dd = lk + 8 - (int)((o >> 16) & 0xff);
+        const int a = li + 8 - (int)(o & 0xff), b = lj + 8 - (int)((o >> 8) & 0xff);
+        const int zo = (int)((o >> 16) & 0xff) - 8;  // window origin relative to the item
+        const int zr = sm.rng[c][2], zl = zr & 0xff, zh = zr >> 8;
         const double4 fw = sm.fw[c][a];
-        const double2 wy = sm.wyz[c][0][b], wz = sm.wyz[c][1][dd];
-        if (!CC) {
-          a1 = a1 + fw.x * wy.y * wz.y * fw.w;
-          a2 = a2 + fw.y * wy.x * wz.y * fw.w;
-          a3 = a3 + fw.z * wy.y * wz.x * fw.w;
-        } else {
-          a1 = a1 + fw.x * wy.y * wz.y * fw.w;
-          a2 = a2 + fw.y * wy.y * wz.y * fw.w;
-          a3 = a3 + fw.z * wy.y * wz.y * fw.w;
+        const double2 wy = sm.wyz[c][0][b];
+        const double p1 = fw.x * wy.y, p2 = CC ? fw.y * wy.y : fw.y * wy.x, p3 = fw.z * wy.y;
+#pragma unroll
+        for (int r = 0; r < OB_CPT; ++r) {
+          if (r < zl || r > zh) continue;
+          const double2 wz = sm.wyz[c][1][r - zo];
+          acc[r][0] = acc[r][0] + p1 * wz.y * fw.w;
+          acc[r][1] = acc[r][1] + p2 * wz.y * fw.w;
+          acc[r][2] = acc[r][2] + p3 * (CC ? wz.y : wz.x) * fw.w;
         }
       }
-      acc[r][0] = a1, acc[r][1] = a2, acc[r][2] = a3;
     }
     __syncthreads();
   }
@@ -1374,22 +1384,22 @@ struct ObjSpread {
       }
     }
     if (npend > 0) chunk(sm, npend, g, klo, khi, x, y, z, f1, f2, f3, area, acc);
-    const int ncell = it.nx * it.ny * it.nz;
-#pragma unroll
-    for (int r = 0; r < OB_CPT; ++r) {
-      const int cell = tid + r * OB_THREADS;
-      if (cell >= ncell) continue;
-      const int li = cell % it.nx, lj = (cell / it.nx) % it.ny, lk = cell / (it.nx * it.ny);
+    if (tid < it.nx * it.ny) {
+      const int li = tid % it.nx, lj = tid / it.nx;
       const int i = it.x0 + li;
-      const long long mm =
-          (long long)i + (long long)(it.y0 + lj) * g.sy + (long long)(it.z0 + lk) * g.sz;
-      // the cell and, for i = itot or i = 1, its periodic x-halo duplicate (bound.f90:141-171:
-      // var(0) = var(imax), var(i1) = var(1)); the y halos are copied later by k_jhalo
-      o1[mm] = acc[r][0], o2[mm] = acc[r][1], o3[mm] = acc[r][2];
-      if (i == g.itot)
-        o1[mm - g.itot] = acc[r][0], o2[mm - g.itot] = acc[r][1], o3[mm - g.itot] = acc[r][2];
-      if (i == 1)
-        o1[mm + g.itot] = acc[r][0], o2[mm + g.itot] = acc[r][1], o3[mm + g.itot] = acc[r][2];
+      const long long m0 = (long long)i + (long long)(it.y0 + lj) * g.sy + (long long)it.z0 * g.sz;
+#pragma unroll
+      for (int r = 0; r < OB_CPT; ++r) {
+        if (r >= it.nz) break;
+        const long long mm = m0 + (long long)r * g.sz;
+        // the cell and, for i = itot or i = 1, its periodic x-halo duplicate (bound.f90:141-171:
+        // var(0) = var(imax), var(i1) = var(1)); the y halos are copied later by k_jhalo
+        o1[mm] = acc[r][0], o2[mm] = acc[r][1], o3[mm] = acc[r][2];
+        if (i == g.itot)
+          o1[mm - g.itot] = acc[r][0], o2[mm - g.itot] = acc[r][1], o3[mm - g.itot] = acc[r][2];
+        if (i == 1)
+          o1[mm + g.itot] = acc[r][0], o2[mm + g.itot] = acc[r][1], o3[mm + g.itot] = acc[r][2];
+      }
     }
   }
 };

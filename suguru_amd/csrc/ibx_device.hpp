@@ -857,9 +857,10 @@ __global__ void __launch_bounds__(SP_THREADS)
 // k_obj_sources (source lists); k_obj_spread (items and zero units); k_jhalo.
 // ---------------------------------------------------------------------------------------
 constexpr int OB_BX = 32;          // item extent limit in x and y (cells)
-constexpr int OB_THREADS = 256;    // also the limit of an item's columns nx * ny
-constexpr int OB_CPT = 8;          // item extent limit in z: one column of cells per thread
-constexpr int OB_NC = OB_THREADS * OB_CPT;  // cells per item
+constexpr int OB_NZ = 32;          // item extent limit in z
+constexpr int OB_NC = 2048;        // cells per item
+constexpr int OB_THREADS = 256;
+constexpr int OB_CPT = OB_NC / OB_THREADS;  // cells per thread
 constexpr int OB_PEND = 64 + OB_THREADS;    // pending candidate list
 constexpr int OB_NSRC = 32;        // source objects per item
 constexpr int OB_ZIT = 8;          // cell pairs per thread in one zero unit
@@ -887,29 +888,17 @@ __global__ void __launch_bounds__(256) k_obj_clear(long long nw, unsigned* __res
 }
 
 // Item grid of one object box: cx x cy x cz sub-boxes of extent nx x ny x nz (the last ones
-// cut short) with nx, ny <= OB_BX, nx * ny <= OB_THREADS columns and nz <= OB_CPT.  Of the
-// few x splits tried (the fewest pieces and up to two more), the one of least estimated
-// work: per item, the cells its candidates' windows can reach, (nx+3)(ny+3)(nz+3), plus a
-// fixed cost for the per-item candidate scan.
+// cut short), at most OB_BX x OB_BX x OB_NZ and OB_NC cells each.
 struct ObjGrid {
   int cx, cy, cz, nx, ny, nz;
   __device__ explicit ObjGrid(const int* b) {
     const int X = b[1] - b[0] + 1, Y = b[3] - b[2] + 1, Z = b[5] - b[4] + 1;
-    nz = min(Z, OB_CPT);
+    cx = (X + OB_BX - 1) / OB_BX, cy = (Y + OB_BX - 1) / OB_BX;
+    nx = (X + cx - 1) / cx, ny = (Y + cy - 1) / cy;
+    nz = max(1, min(min(Z, OB_NZ), OB_NC / (nx * ny)));
     cz = (Z + nz - 1) / nz;
-    long long best = -1;
-    const int a0 = (X + OB_BX - 1) / OB_BX;
-    for (int ax = a0; ax <= min(X, a0 + 2); ++ax) {
-      const int ex = (X + ax - 1) / ax;
-      const int ay = (Y + min(OB_BX, OB_THREADS / ex) - 1) / min(OB_BX, OB_THREADS / ex);
-      const int ey = (Y + ay - 1) / ay;
-      const long long cost =
-          (long long)ax * ay * cz * ((long long)(ex + 3) * (ey + 3) * (nz + 3) + kItemCost);
-      if (best < 0 || cost < best) best = cost, cx = ax, cy = ay, nx = ex, ny = ey;
-    }
   }
   __device__ int count() const { return cx * cy * cz; }
-  static constexpr long long kItemCost = 1000;
 };
 
 // Coarse bins of OB_BIN^3 cells over i in [1,itot], j in [1,jtot], k in [0,k1]: bin mask
@@ -1202,10 +1191,38 @@ __global__ void __launch_bounds__(256) k_obj_sources(int nwd, const int* __restr
   }
 }
 
+__device__ __forceinline__ bool obj_covered(const unsigned* __restrict__ cover, long long m) {
+  return (cover[m >> 5] >> (m & 31)) & 1u;
+}
+
+// Zeros of the uncovered cells of the 16-cell group (linear index) that holds cell mc: a
+// cell is covered when its bitmap bit is set, a halo cell i = 0 / i1 when the bit of its
+// source itot / 1 is (the items write those duplicates).  The group's 16 bits lie in one
+// bitmap word; a group holds at most one cell i = 0 and one i = i1 (i1 + 1 > 16).
+__device__ __forceinline__ void obj_pad_group(const GridD& g, long long ncells,
+                                              const unsigned* __restrict__ cover, long long mc,
+                                              double* __restrict__ o1, double* __restrict__ o2,
+                                              double* __restrict__ o3) {
+  const long long g0 = mc & ~15LL;
+  unsigned bits = (cover[g0 >> 5] >> (g0 & 16)) & 0xffffu;
+  const int i0 = (int)(g0 % g.sy);
+  const int e0 = i0 == 0 ? 0 : (int)g.sy - i0;  // offset of the cell i = 0
+  const int e1 = (int)g.sy - 1 - i0;             // offset of the cell i = i1
+  if (e0 < 16 && g0 + e0 < ncells)
+    bits = (bits & ~(1u << e0)) | ((unsigned)obj_covered(cover, g0 + e0 + g.itot) << e0);
+  if (e1 < 16 && g0 + e1 < ncells)
+    bits = (bits & ~(1u << e1)) | ((unsigned)obj_covered(cover, g0 + e1 - g.itot) << e1);
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const long long m = g0 + e;
+    if (!((bits >> e) & 1u) && m < ncells) o1[m] = 0.0, o2[m] = 0.0, o3[m] = 0.0;
+  }
+}
+
 template <bool CC>
 struct ObjSpread {
   struct Smem {
-    unsigned long long mx[OB_BX], my[OB_BX];  // candidates whose window covers x / y
+    unsigned long long mx[OB_BX], my[OB_BX], mz[OB_NZ];  // candidates covering x / y / z
     int pend[OB_PEND];          // point index | nb << OB_QBITS
     double4 fw[SP_CH][4];       // per candidate and x tap: f1*wx, f2*wx, f3*wx, area/dVeul
     double2 wyz[SP_CH][2][4];   // per candidate, y / z tap: (centred, staggered) weight
@@ -1225,12 +1242,16 @@ struct ObjSpread {
                                const double* __restrict__ area, double (&acc)[OB_CPT][3]) {
     const int tid = threadIdx.x;
     const ObjItem& it = sm.it;
+    const int ncell = it.nx * it.ny * it.nz;
     for (int c = tid >> 2; c < m; c += OB_THREADS / 4) {
       const int d = tid & 3;
       if (d == 3) continue;
       const int e = sm.pend[c];
       const long long q = e & (int)(OB_MAXPTS - 1);
       const int s = nb_slot(e >> OB_QBITS);
+      const double v1 = f1[q], v2 = f2[q], v3 = f3[q];
+      const double adv = area[q] / g.dveul;
+      const bool fin = isfinite(v1) && isfinite(v2) && isfinite(v3) && isfinite(adv);
       double cc_;
       int lo, hi, base_, ext;
       if (d == 0) {
@@ -1255,13 +1276,16 @@ struct ObjSpread {
         ws[1] = ax.st[0], ws[2] = ax.st[1], ws[3] = ax.st[2];
         int a, b;
         window(cc_, lo, hi, w0, a, b);
+        // tap w0 has weight 0 in all three components (ws[0] = 0, and wn[0] = 0 unless the
+        // centred stencil starts there; the cell-centred form uses ws only), so its terms are
+        // +-0 and leave every sum unchanged: it is left out, which is exact for a finite
+        // force and area (a non-finite one keeps the full window, as in the reference loop)
+        if ((CC || !n0) && a == w0 && fin) ++a;
         w0 -= base_;
         bl = max(a - base_, 0);
         bh = min(b - base_, ext - 1);
       }
       if (d == 0) {
-        const double v1 = f1[q], v2 = f2[q], v3 = f3[q];
-        const double adv = area[q] / g.dveul;
 #pragma unroll
         for (int u = 0; u < 4; ++u)
           sm.fw[c][u] = CC ? make_double4(v1 * ws[u], v2 * ws[u], v3 * ws[u], adv)
@@ -1274,15 +1298,14 @@ struct ObjSpread {
       reinterpret_cast<unsigned char*>(&sm.org[c])[d] = (unsigned char)(w0 + 8);
     }
     __syncthreads();
-    // separable masks: candidate c touches column (li, lj) iff its clamped window covers li and
-    // lj, so the column's mask is mx[li] & my[lj]; wavefront w < 2 builds the masks of
-    // direction w with one ballot per position (lane = candidate).  Every candidate's window
-    // meets the item's z range.
+    // separable masks: candidate c touches cell (li, lj, lk) iff its clamped window covers li,
+    // lj and lk, so the cell's mask is mx[li] & my[lj] & mz[lk]; wavefront w < 3 builds the
+    // masks of direction w with one ballot per position (lane = candidate)
     {
       const int wv = tid >> 6, lane = tid & 63;
-      if (wv < 2) {
-        const int ext = wv == 0 ? it.nx : it.ny;
-        unsigned long long* dst = wv == 0 ? sm.mx : sm.my;
+      if (wv < 3) {
+        const int ext = wv == 0 ? it.nx : (wv == 1 ? it.ny : it.nz);
+        unsigned long long* dst = wv == 0 ? sm.mx : (wv == 1 ? sm.my : sm.mz);
         const int r = lane < m ? sm.rng[lane][wv] : 0x00ff;  // a = 255 > b: covers nothing
         const int a = r & 0xff, b = r >> 8;
         for (int l = 0; l < ext; ++l) {
@@ -1292,32 +1315,33 @@ struct ObjSpread {
       }
     }
     __syncthreads();
-    // ordered accumulation: thread = column (li, lj), its cells lk = 0..nz-1 in acc[lk]; bits
-    // ascending = pending order = K order.  The term ((f wx) wy) wz (area/dVeul) keeps the
-    // reference's order of products; its first two factors are formed once per candidate and
-    // column, then multiplied by the z tap of each cell in the clamped z window.
-    if (tid < it.nx * it.ny) {
-      const int li = tid % it.nx, lj = tid / it.nx;
-      unsigned long long mk = sm.mx[li] & sm.my[lj];
+    // ordered accumulation: bits ascending = pending order = K order
+#pragma unroll
+    for (int r = 0; r < OB_CPT; ++r) {
+      const int cell = tid + r * OB_THREADS;
+      if (cell >= ncell) continue;
+      const int li = cell % it.nx, lj = (cell / it.nx) % it.ny, lk = cell / (it.nx * it.ny);
+      unsigned long long mk = sm.mx[li] & sm.my[lj] & sm.mz[lk];
+      double a1 = acc[r][0], a2 = acc[r][1], a3 = acc[r][2];
       while (mk) {
         const int c = __builtin_ctzll(mk);
         mk &= mk - 1;
         const unsigned o = (unsigned)sm.org[c];
-        const int a = li + 8 - (int)(o & 0xff), b = lj + 8 - (int)((o >> 8) & 0xff);
-        const int zo = (int)((o >> 16) & 0xff) - 8;  // window origin relative to the item
-        const int zr = sm.rng[c][2], zl = zr & 0xff, zh = zr >> 8;
+        const int a = li + 8 - (int)(o & 0xff), b = lj + 8 - (int)((o >> 8) & 0xff),
+                  dd = lk + 8 - (int)((o >> 16) & 0xff);
         const double4 fw = sm.fw[c][a];
-        const double2 wy = sm.wyz[c][0][b];
-        const double p1 = fw.x * wy.y, p2 = CC ? fw.y * wy.y : fw.y * wy.x, p3 = fw.z * wy.y;
-#pragma unroll
-        for (int r = 0; r < OB_CPT; ++r) {
-          if (r < zl || r > zh) continue;
-          const double2 wz = sm.wyz[c][1][r - zo];
-          acc[r][0] = acc[r][0] + p1 * wz.y * fw.w;
-          acc[r][1] = acc[r][1] + p2 * wz.y * fw.w;
-          acc[r][2] = acc[r][2] + p3 * (CC ? wz.y : wz.x) * fw.w;
+        const double2 wy = sm.wyz[c][0][b], wz = sm.wyz[c][1][dd];
+        if (!CC) {
+          a1 = a1 + fw.x * wy.y * wz.y * fw.w;
+          a2 = a2 + fw.y * wy.x * wz.y * fw.w;
+          a3 = a3 + fw.z * wy.y * wz.x * fw.w;
+        } else {
+          a1 = a1 + fw.x * wy.y * wz.y * fw.w;
+          a2 = a2 + fw.y * wy.y * wz.y * fw.w;
+          a3 = a3 + fw.z * wy.y * wz.y * fw.w;
         }
       }
+      acc[r][0] = a1, acc[r][1] = a2, acc[r][2] = a3;
     }
     __syncthreads();
   }
@@ -1330,8 +1354,10 @@ struct ObjSpread {
                               const double* __restrict__ x, const double* __restrict__ y,
                               const double* __restrict__ z, const double* __restrict__ f1,
                               const double* __restrict__ f2, const double* __restrict__ f3,
-                              const double* __restrict__ area, double* __restrict__ o1,
-                              double* __restrict__ o2, double* __restrict__ o3) {
+                              const double* __restrict__ area,
+                              const unsigned* __restrict__ cover, long long ncells,
+                              double* __restrict__ o1, double* __restrict__ o2,
+                              double* __restrict__ o3) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const ObjItem it = sm.it;
     double acc[OB_CPT][3];
@@ -1384,34 +1410,48 @@ struct ObjSpread {
       }
     }
     if (npend > 0) chunk(sm, npend, g, klo, khi, x, y, z, f1, f2, f3, area, acc);
-    if (tid < it.nx * it.ny) {
-      const int li = tid % it.nx, lj = tid / it.nx;
-      const int i = it.x0 + li;
-      const long long m0 = (long long)i + (long long)(it.y0 + lj) * g.sy + (long long)it.z0 * g.sz;
+    const int ncell = it.nx * it.ny * it.nz;
 #pragma unroll
-      for (int r = 0; r < OB_CPT; ++r) {
-        if (r >= it.nz) break;
-        const long long mm = m0 + (long long)r * g.sz;
-        // the cell and, for i = itot or i = 1, its periodic x-halo duplicate (bound.f90:141-171:
-        // var(0) = var(imax), var(i1) = var(1)); the y halos are copied later by k_jhalo
-        o1[mm] = acc[r][0], o2[mm] = acc[r][1], o3[mm] = acc[r][2];
-        if (i == g.itot)
-          o1[mm - g.itot] = acc[r][0], o2[mm - g.itot] = acc[r][1], o3[mm - g.itot] = acc[r][2];
-        if (i == 1)
-          o1[mm + g.itot] = acc[r][0], o2[mm + g.itot] = acc[r][1], o3[mm + g.itot] = acc[r][2];
-      }
+    for (int r = 0; r < OB_CPT; ++r) {
+      const int cell = tid + r * OB_THREADS;
+      if (cell >= ncell) continue;
+      const int li = cell % it.nx, lj = (cell / it.nx) % it.ny, lk = cell / (it.nx * it.ny);
+      const int i = it.x0 + li;
+      const long long mm =
+          (long long)i + (long long)(it.y0 + lj) * g.sy + (long long)(it.z0 + lk) * g.sz;
+      // the cell and, for i = itot or i = 1, its periodic x-halo duplicate (bound.f90:141-171:
+      // var(0) = var(imax), var(i1) = var(1)); the y halos are copied later by k_jhalo
+      o1[mm] = acc[r][0], o2[mm] = acc[r][1], o3[mm] = acc[r][2];
+      if (i == g.itot)
+        o1[mm - g.itot] = acc[r][0], o2[mm - g.itot] = acc[r][1], o3[mm - g.itot] = acc[r][2];
+      if (i == 1)
+        o1[mm + g.itot] = acc[r][0], o2[mm + g.itot] = acc[r][1], o3[mm + g.itot] = acc[r][2];
+    }
+    // the uncovered cells of the 16-cell groups this item writes in part (zero units skip
+    // every group that holds a covered cell): per row of the box, the groups of its first
+    // and last cell and of its x-halo duplicates; the groups between lie inside the box
+    const int rows = it.ny * it.nz;
+    const bool h0 = it.x0 + it.nx - 1 == g.itot, h1 = it.x0 == 1;
+    for (int t = tid; t < 4 * rows; t += OB_THREADS) {
+      const int r = t >> 2, slot = t & 3;
+      if ((slot == 2 && !h0) || (slot == 3 && !h1)) continue;
+      const long long rb =
+          (long long)(it.y0 + r % it.ny) * g.sy + (long long)(it.z0 + r / it.ny) * g.sz;
+      const long long mc = slot == 0   ? rb + it.x0
+                           : slot == 1 ? rb + it.x0 + it.nx - 1
+                           : slot == 2 ? rb
+                                       : rb + g.itot + 1;
+      obj_pad_group(g, ncells, cover, mc, o1, o2, o3);
     }
   }
 };
 
-__device__ __forceinline__ bool obj_covered(const unsigned* __restrict__ cover, long long m) {
-  return (cover[m >> 5] >> (m & 31)) & 1u;
-}
-
 // Zero unit u: cells [u OB_ZCELLS, (u+1) OB_ZCELLS) of the linear index, as pairs
-// (2 tid + 2 OB_THREADS e, +1): every cell that no item writes gets 0, i.e. every cell whose
-// bitmap bit is clear -- for the halo cells i = 0 and i1, the bit of their sources itot and 1
-// (the items write those duplicates).  All bitmap words are read before the stores.
+// (2 tid + 2 OB_THREADS e, +1), eight lanes per 16-cell group: the cells of every group
+// without a covered cell get 0 (covered: the bitmap bit is set; for the halo cells i = 0 and
+// i1, the bit of their sources itot and 1, whose duplicates the items write).  Groups with a
+// covered cell are completed by the items, so that every 128-byte line is written whole by
+// one side.  All bitmap words are read before the stores.
 __device__ __forceinline__ void obj_zero_unit(const GridD& g, long long ncells, long long u,
                                               const unsigned* __restrict__ cover,
                                               double* __restrict__ o1,
@@ -1427,8 +1467,13 @@ __device__ __forceinline__ void obj_zero_unit(const GridD& g, long long ncells, 
     const int i1 = i + 1 == sy ? 0 : i + 1;
     const long long sa = i == 0 ? m + g.itot : (i == sy - 1 ? m - g.itot : m);
     const long long sb = i1 == 0 ? m + 1 + g.itot : (i1 == sy - 1 ? m + 1 - g.itot : m + 1);
-    za[e] = m < ncells && !obj_covered(cover, sa);
-    zb[e] = m + 1 < ncells && !obj_covered(cover, sb);
+    // whole 16-cell groups only (8 lanes): a group with a covered cell is left to the items
+    const bool ca = m < ncells && obj_covered(cover, sa);
+    const bool cb = m + 1 < ncells && obj_covered(cover, sb);
+    const unsigned long long bal = __ballot(ca || cb);
+    const bool any = ((bal >> (threadIdx.x & 56)) & 0xffull) != 0ull;
+    za[e] = m < ncells && !any;
+    zb[e] = m + 1 < ncells && !any;
     i += step;
     if (i >= sy) i -= sy;
   }
@@ -1485,7 +1530,8 @@ __global__ void __launch_bounds__(OB_THREADS, MINW)
       if (item < nitems) {
         if (tid < sm.ns) sm.src[tid] = src[(long long)item * OB_NSRC + tid];
         __syncthreads();
-        O::item(sm, g, klo, khi, cap_off, x, y, z, f1, f2, f3, area, o1, o2, o3);
+        O::item(sm, g, klo, khi, cap_off, x, y, z, f1, f2, f3, area, cover, ncells, o1, o2,
+                o3);
       } else {
         items_left = false;
       }

@@ -10,8 +10,9 @@ capsules of radius U(4h,10h) and degree 8..24 by radius (1.16 M Lagrangian point
     python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--workload uneven|moderate|small]
 
 Prints one JSON line.  `value` = Lagrangian points per second over interp + spread.
-`roofline` is for the dominant kernel (the tile spread kernel, which writes the whole force
-field); `path_roofline` is the same figure for the whole interp + spread step.
+`roofline` is for the dominant kernel, the spread's main kernel (k_obj_spread for the
+capsule-object configurations, k_spread for the tile ones), which writes the whole force
+field; `path_roofline` is the same figure for the whole interp + spread step.
 """
 from __future__ import annotations
 
@@ -108,6 +109,9 @@ def main():
                     help="interpolation on the same stream as the spread (default: concurrent)")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + steps only (for rocprofv3), no extras")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device three times (the reference kernel "
+                         "for scaling rocprofv3's FETCH_SIZE and WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU only; the path's multi-GPU sharding is "
@@ -115,6 +119,15 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_calibration:
+        a = torch.ones(2 ** 27, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copied_bytes": a.numel() * 8, "copies": 3}))
+        return
 
     from suguru_amd import Batch, IBTransfer, workload as W
 
@@ -190,7 +203,9 @@ def main():
     stage = {k: float(np.mean([s[k] for s in slots])) for k in Batch.SLOT_KEYS}
     tile_s = stage["tile_spread_us"] * 1e-6
     achieved = b_spread / tile_s / 1e9
-    traffic = read_traffic(f"traffic_spread_tile_{args.workload}.json")
+    cfg_name = Batch.config_names()[choice]
+    objects = cfg_name.startswith("objects")
+    traffic = read_traffic(f"traffic_{cfg_name}_{args.workload}.json")
     out = {
         "metric": "IB interp+spread Lag-pts/s & HBM GB/s; step time surrogate vs plain @1e-3",
         "value": n / (ms * 1e-3),
@@ -217,7 +232,10 @@ def main():
             "interp_spread_overlap": not args.no_overlap,
         },
         "roofline": {
-            "kernel": "k_spread (tile spread, writes the full 3-component force field)",
+            "kernel": ("k_obj_spread (capsule-object spread: ordered per-cell sums of the "
+                       "work items plus the zero units; writes the full 3-component force "
+                       "field)") if objects else
+                      "k_spread (tile spread, writes the full 3-component force field)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

@@ -219,11 +219,11 @@ int enqueue_spread_obj(ibx_ctx* c, const TileCfg& cf, bool cc, long long ncap,
       (rc = ensure(c, c->osrc, (size_t)icap * OB_NSRC * sizeof(int))) ||
       (rc = ensure(c, c->onsrc, (size_t)icap * sizeof(int))) ||
       (rc = ensure(c, c->ocover, (size_t)nw * sizeof(unsigned))) ||
-      (rc = ensure(c, c->ometa, 8 * sizeof(int))))
+      (rc = ensure(c, c->ometa, OB_META * sizeof(int))))
     return rc;
   int* meta = (int*)c->ometa.p;
   unsigned* cover = (unsigned*)c->ocover.p;
-  k_obj_clear<<<blocks_for(std::max<long long>(std::max(nw, nbw), 8), 256), 256, 0,
+  k_obj_clear<<<blocks_for(std::max<long long>(std::max(nw, nbw), OB_META), 256), 256, 0,
                 c->stream>>>(nw, cover, nbw, (unsigned long long*)c->obinm.p, meta);
   if (npts > 0 && ncap > 0) {
     k_obj_boxes<<<(unsigned)ncap, 256, 0, c->stream>>>(

@@ -872,7 +872,10 @@ struct ObjItem {
   int p, nb, x0, y0, z0, nx, ny, nz;
 };
 
-// counters in `meta`: [0] items, [2] item queue, [3] zero-unit queue
+// counters in `meta` (OB_META ints): [0] items; [OB_QSTRIDE * (1 + x)] the item queue of
+// XCD x = 0..7 (its own 128-byte line), which serves items [x n / 8, (x + 1) n / 8)
+constexpr int OB_QSTRIDE = 32;
+constexpr int OB_META = OB_QSTRIDE * 9;
 // boxes: [nobj][6] clamped window bounds xa, xb, ya, yb, za, zb of object o = 9 p + nb
 // (xa > xb: no candidate); ubox: [ncap][6] their union per capsule
 
@@ -884,7 +887,7 @@ __global__ void __launch_bounds__(256) k_obj_clear(long long nw, unsigned* __res
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < nw) cover[q] = 0u;
   if (q < nbw) binm[q] = 0ull;
-  if (q < 8) meta[q] = 0;
+  if (q < OB_META) meta[q] = 0;
 }
 
 // Item grid of one object box: cx x cy x cz sub-boxes of extent nx x ny x nz (the last ones
@@ -1492,8 +1495,7 @@ __device__ __forceinline__ void obj_zero_unit(const GridD& g, long long ncells, 
   }
 }
 
-// Persistent.  Each workgroup alternates: one item from the item queue meta[2], then up to
-// `per` of its own zero units (units blockIdx.x, blockIdx.x + gridDim.x, ...: a static
+// Persistent.  Each workgroup alternates: one item, then up to `per` of its own zero units (units blockIdx.x, blockIdx.x + gridDim.x, ...: a static
 // share, no counter to contend for); once the items are gone, the rest of its units.  Items
 // and zero units write disjoint cells, so their order does not matter.  MINW: the minimum wavefronts
 // per SIMD the register allocation must allow (1: unconstrained).
@@ -1517,17 +1519,31 @@ __global__ void __launch_bounds__(OB_THREADS, MINW)
   const long long mine = nzu > blockIdx.x ? (nzu - blockIdx.x + gridDim.x - 1) / gridDim.x : 0;
   const int per = nitems > 0 ? (int)((mine * gridDim.x + nitems - 1) / nitems) : 0;
   long long zu = blockIdx.x;
+  const int xcd = blockIdx.x & 7;
+  int qi = 0;  // queues tried so far (thread 0's view)
   bool items_left = nitems > 0, zero_left = mine > 0;
   while (items_left || zero_left) {
     if (items_left) {
+      // items: from the queue of this workgroup's XCD (a contiguous share, so that a
+      // capsule's items and their points stay in one L2), then from the other XCDs' queues
       if (tid == 0) {
-        const int i = atomicAdd(&meta[2], 1);
-        sm.item = i;
-        if (i < nitems) sm.it = items[i], sm.ns = nsrc[i];
+        int item = -1;
+        while (qi < 8) {
+          const int x = (xcd + qi) & 7;
+          const int lo = (int)((long long)x * nitems / 8), hi = (int)((long long)(x + 1) * nitems / 8);
+          const int i = lo + atomicAdd(&meta[OB_QSTRIDE * (1 + x)], 1);
+          if (i < hi) {
+            item = i;
+            break;
+          }
+          ++qi;
+        }
+        sm.item = item;
+        if (item >= 0) sm.it = items[item], sm.ns = nsrc[item];
       }
       __syncthreads();
       const int item = sm.item;
-      if (item < nitems) {
+      if (item >= 0) {
         if (tid < sm.ns) sm.src[tid] = src[(long long)item * OB_NSRC + tid];
         __syncthreads();
         O::item(sm, g, klo, khi, cap_off, x, y, z, f1, f2, f3, area, cover, ncells, o1, o2,

@@ -12,7 +12,10 @@ capsules of radius U(4h,10h) and degree 8..24 by radius (1.16 M Lagrangian point
 Prints one JSON line.  `value` = Lagrangian points per second over interp + spread.
 `roofline` is for the dominant kernel, the spread's main kernel (k_obj_spread for the
 capsule-object configurations, k_spread for the tile ones), which writes the whole force
-field; `path_roofline` is the same figure for the whole interp + spread step.
+field; `path_roofline` is the same figure for the whole interp + spread step.  `surrogate`
+compares the step time with the learned membrane traction against the plain FP64 law
+(`--no-surrogate` skips that leg), and `cpu_baseline` times the CPU oracle on the same
+workload (`--no-cpu` skips it).
 """
 from __future__ import annotations
 
@@ -85,6 +88,29 @@ def cpu_baseline(wl):
     }
 
 
+def surrogate_leg(transfer_ms):
+    """The learned membrane traction at degree 16 (289 SH coefficients x 3 -> traction at 578
+    points x 3; 4 hidden layers of 512, SiLU; BF16 MFMA, FP32 accumulation): trained here on
+    200 000 seeded shapes labelled by the FP64 law on the GPU, then timed on 2048 held-out
+    capsules against the plain FP64 law, both with the measured interp + spread time added
+    as the rest of the step.  The surrogate step keeps a capsule's prediction only where its
+    equilibrium residual is within tol = 1e-3 and runs the FP64 law for the others."""
+    from suguru_amd.surrogate import run_experiment
+
+    r = run_experiment(n_train=200_000, n_test=2048, steps=20_000, batch=512, lr=1e-3,
+                       seed=1, tol=1e-3)
+    keep = ("n_train", "steps", "train_s", "rel_err_untrained_median", "rel_err_median",
+            "rel_err_p90", "indicator_median", "tol", "fallback_fraction", "final_rel_err_max",
+            "t_law_ms", "t_surrogate_step_ms")
+    out = {k: r[k] for k in keep}
+    out["capsules"] = 2048
+    out["plain_step_ms"] = r["t_law_ms"] + transfer_ms
+    out["surrogate_step_ms"] = r["t_surrogate_step_ms"] + transfer_ms
+    out["speedup"] = out["plain_step_ms"] / out["surrogate_step_ms"]
+    out["transfer_ms_added"] = transfer_ms
+    return out
+
+
 def read_traffic(name):
     """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE), committed
     under profiles/ as JSON; None if absent."""
@@ -109,6 +135,8 @@ def main():
                     help="interpolation on the same stream as the spread (default: concurrent)")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + steps only (for rocprofv3), no extras")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the learned-traction leg (training + step-time comparison)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times (the reference kernel "
                          "for scaling rocprofv3's FETCH_SIZE and WRITE_SIZE counters)")
@@ -259,6 +287,11 @@ def main():
         },
         "setup_s": round(t_setup, 2),
     }
+    if not args.no_surrogate:
+        try:
+            out["surrogate"] = surrogate_leg(ms)
+        except Exception as e:  # reported, never hides the transfer line
+            out["surrogate"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

@@ -101,7 +101,7 @@ def surrogate_leg(transfer_ms):
                        seed=1, tol=1e-3)
     keep = ("n_train", "steps", "train_s", "rel_err_untrained_median", "rel_err_median",
             "rel_err_p90", "indicator_median", "tol", "fallback_fraction", "final_rel_err_max",
-            "t_law_ms", "t_surrogate_step_ms")
+            "t_law_ms", "t_surrogate_step_ms", "t_inference_ms", "tol_sweep")
     out = {k: r[k] for k in keep}
     out["capsules"] = 2048
     out["plain_step_ms"] = r["t_law_ms"] + transfer_ms

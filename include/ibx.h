@@ -56,7 +56,8 @@ extern "C" {
 #define IBX_OK 0
 #define IBX_E_ARG (-1)      /* invalid argument (null pointer, bad size, bad offsets) */
 #define IBX_E_HIP (-2)      /* HIP runtime error */
-#define IBX_E_DOMAIN (-3)   /* non-finite position, or a point beyond the one-image range */
+#define IBX_E_DOMAIN (-3)   /* non-finite position (or, in a spread, force or area), or a
+                               point beyond the one-image range */
 #define IBX_E_CAPACITY (-4) /* internal candidate list overflow in a captured graph: re-plan */
 #define IBX_E_STATE (-5)    /* call out of order (e.g. batch run before bind) */
 
@@ -208,6 +209,8 @@ int ibx_batch_slot_times(ibx_batch* b, double* us);
  * their source objects minus one), [8] 0. */
 #define IBX_PROF_N 9
 int ibx_batch_profile(ibx_batch* b, int reps, double* us);
+/* Also done by ibx_destroy for every batch of the context that is still alive; a batch
+ * must not be used or destroyed after its context. */
 void ibx_batch_destroy(ibx_batch* b);
 
 #ifdef __cplusplus

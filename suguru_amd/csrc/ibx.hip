@@ -81,11 +81,12 @@ struct ibx_ctx {
   // spread binning
   DevBuf tcount, toff, tpart, tlist, tmeta, entries;
   // spread by capsule objects
-  DevBuf oboxes, oubox, oitems, osrc, onsrc, ocover, ometa, onbm, ocnt, ooff, obinm;
+  DevBuf oboxes, oitems, osrc, onsrc, ocover, ometa, onbm, ocnt, ooff, obinm;
   // staging for the host-array entry points
   DevBuf s_f1, s_f2, s_f3, s_x, s_y, s_z, s_a1, s_a2, s_a3, s_a4, s_o1, s_o2, s_o3, s_off;
   int cfg = 0;
   unsigned long long ecap = 0;  // entries capacity (elements)
+  std::vector<ibx_batch*> batches;  // live batches, destroyed with the context
 };
 
 struct ibx_batch {
@@ -214,7 +215,6 @@ int enqueue_spread_obj(ibx_ctx* c, const TileCfg& cf, bool cc, long long ncap,
       (rc = ensure(c, c->ocnt, (size_t)std::max<long long>(1, ncap) * sizeof(int))) ||
       (rc = ensure(c, c->obinm, (size_t)std::max<long long>(1, nbw) * 8)) ||
       (rc = ensure(c, c->ooff, (size_t)std::max<long long>(1, ncap) * sizeof(int))) ||
-      (rc = ensure(c, c->oubox, (size_t)std::max<long long>(1, ncap) * 6 * sizeof(int))) ||
       (rc = ensure(c, c->oitems, (size_t)icap * sizeof(ObjItem))) ||
       (rc = ensure(c, c->osrc, (size_t)icap * OB_NSRC * sizeof(int))) ||
       (rc = ensure(c, c->onsrc, (size_t)icap * sizeof(int))) ||
@@ -227,7 +227,7 @@ int enqueue_spread_obj(ibx_ctx* c, const TileCfg& cf, bool cc, long long ncap,
                 c->stream>>>(nw, cover, nbw, (unsigned long long*)c->obinm.p, meta);
   if (npts > 0 && ncap > 0) {
     k_obj_boxes<<<(unsigned)ncap, 256, 0, c->stream>>>(
-        c->gd, klo, khi, d_off, x, y, z, (int*)c->oboxes.p, (int*)c->oubox.p, (int*)c->ocnt.p,
+        c->gd, klo, khi, d_off, x, y, z, (int*)c->oboxes.p, (int*)c->ocnt.p,
         nwd, (unsigned long long*)c->obinm.p, c->d_err);
     k_obj_scan<<<1, 1024, 0, c->stream>>>(ncap, (const int*)c->ocnt.p, (int*)c->ooff.p,
                                           (int)icap, meta, c->d_err);
@@ -246,7 +246,7 @@ int enqueue_spread_obj(ibx_ctx* c, const TileCfg& cf, bool cc, long long ncap,
 #define OBJ_ARGS                                                                             \
   c->gd, klo, khi, d_off, x, y, z, f1, f2, f3, area, (const ObjItem*)c->oitems.p, (int)icap, \
       (const int*)c->osrc.p, (const int*)c->onsrc.p, (const unsigned*)cover, ncells, meta, o1, \
-      o2, o3
+      o2, o3, c->d_err
     if (cf.minw == 4) {
       if (cc)
         k_obj_spread<true, 4><<<nblk, OB_THREADS, 0, c->stream>>>(OBJ_ARGS);
@@ -293,6 +293,9 @@ int enqueue_spread(ibx_ctx* c, int cfgid, bool cc, long long ncap, const long lo
     if ((rc = ensure(c, c->entries, 1024 * sizeof(unsigned long long)))) return rc;
     c->ecap = c->entries.bytes / sizeof(unsigned long long);
   }
+  if (npts > 0)
+    k_check_forces<<<(unsigned)std::min<long long>(blocks_for(npts, 256), 4LL * c->num_cus), 256,
+                     0, c->stream>>>(npts, f1, f2, f3, area, c->d_err);
   k_fill_u32<<<blocks_for(tg.ntiles, 256), 256, 0, c->stream>>>(cnt, tg.ntiles, 0u);
   if (npts > 0)
     k_bin<0><<<blocks_for(npts, 256), 256, 0, c->stream>>>(
@@ -310,6 +313,9 @@ int enqueue_spread(ibx_ctx* c, int cfgid, bool cc, long long ncap, const long lo
       c->ecap = c->entries.bytes / sizeof(unsigned long long);
     }
   }
+  if (npts > 0)
+    k_check_forces<<<(unsigned)std::min<long long>(blocks_for(npts, 256), 4LL * c->num_cus), 256,
+                     0, c->stream>>>(npts, f1, f2, f3, area, c->d_err);
   k_fill_u32<<<blocks_for(tg.ntiles, 256), 256, 0, c->stream>>>(cnt, tg.ntiles, 0u);
   if (npts > 0)
     k_bin<1><<<blocks_for(npts, 256), 256, 0, c->stream>>>(
@@ -367,6 +373,7 @@ int collect_errors(ibx_ctx* c) {
     k_fill_i32<<<1, 64, 0, c->stream>>>(c->d_err, 1, 0);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (h & ERR_NONFINITE) return fail(c, IBX_E_DOMAIN, "non-finite Lagrangian position");
+    if (h & ERR_FORCE) return fail(c, IBX_E_DOMAIN, "non-finite Lagrangian force or area");
     if (h & ERR_CAPACITY)
       return fail(c, IBX_E_CAPACITY, "candidate list capacity exceeded; re-plan the batch");
   }
@@ -447,9 +454,10 @@ int ibx_create(ibx_ctx** out, const ibx_grid* g, int device) {
 
 void ibx_destroy(ibx_ctx* c) {
   if (!c) return;
+  while (!c->batches.empty()) ibx_batch_destroy(c->batches.back());
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  DevBuf* all[] = {&c->oboxes, &c->oubox, &c->oitems, &c->osrc, &c->onsrc, &c->ocover, &c->ometa, &c->onbm, &c->ocnt, &c->ooff, &c->obinm,
+  DevBuf* all[] = {&c->oboxes, &c->oitems, &c->osrc, &c->onsrc, &c->ocover, &c->ometa, &c->onbm, &c->ocnt, &c->ooff, &c->obinm,
                    &c->tcount, &c->toff, &c->tpart, &c->tlist, &c->tmeta, &c->entries, &c->s_f1, &c->s_f2,
                    &c->s_f3,   &c->s_x,  &c->s_y,   &c->s_z,     &c->s_a1, &c->s_a2,
                    &c->s_a3,   &c->s_a4, &c->s_o1,  &c->s_o2,    &c->s_o3, &c->s_off};
@@ -707,6 +715,7 @@ int ibx_batch_create(ibx_ctx* c, ibx_batch** out, int64_t ncap, const int64_t* c
     return rc;
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->batches.push_back(b);
   *out = b;
   return IBX_OK;
 }
@@ -1024,8 +1033,12 @@ int ibx_batch_autotune(ibx_batch* b, const char* cache_path, int* choice) {
 
 void ibx_batch_destroy(ibx_batch* b) {
   if (!b) return;
-  if (b->ctx) (void)hipSetDevice(b->ctx->device);
-  if (b->ctx && b->ctx->stream) (void)hipStreamSynchronize(b->ctx->stream);
+  if (b->ctx) {
+    (void)hipSetDevice(b->ctx->device);
+    if (b->ctx->stream) (void)hipStreamSynchronize(b->ctx->stream);
+    auto& v = b->ctx->batches;
+    v.erase(std::remove(v.begin(), v.end(), b), v.end());
+  }
   drop_graph(b);
   release(b->d_off);
   delete b;

@@ -32,7 +32,7 @@ struct TileGeom {
   long long ntiles;
 };
 
-enum : int { ERR_NONFINITE = 1, ERR_CAPACITY = 2 };
+enum : int { ERR_NONFINITE = 1, ERR_CAPACITY = 2, ERR_FORCE = 4 };
 
 
 // kernel.f90:7-25, in the reference's operation order.  Both branches are evaluated and one
@@ -877,7 +877,7 @@ struct ObjItem {
 constexpr int OB_QSTRIDE = 32;
 constexpr int OB_META = OB_QSTRIDE * 9;
 // boxes: [nobj][6] clamped window bounds xa, xb, ya, yb, za, zb of object o = 9 p + nb
-// (xa > xb: no candidate); ubox: [ncap][6] their union per capsule
+// (xa > xb: no candidate)
 
 // the coverage bitmap (bit = linear cell index), the bin masks and the counters
 __global__ void __launch_bounds__(256) k_obj_clear(long long nw, unsigned* __restrict__ cover,
@@ -924,14 +924,13 @@ struct ObjBins {
 };
 
 // One workgroup per capsule: the clamped window boxes of its nine objects (boxes[o], o =
-// 9 p + nb), their union (ubox[p]), and its number of items (cnt[p]).
+// 9 p + nb), its registration in the coarse bins, and its number of items (cnt[p]).
 __global__ void __launch_bounds__(256) k_obj_boxes(GridD g, int klo, int khi,
                                                    const long long* __restrict__ cap_off,
                                                    const double* __restrict__ x,
                                                    const double* __restrict__ y,
                                                    const double* __restrict__ z,
                                                    int* __restrict__ boxes,
-                                                   int* __restrict__ ubox,
                                                    int* __restrict__ cnt, int nwd,
                                                    unsigned long long* __restrict__ binm,
                                                    int* __restrict__ err) {
@@ -1013,15 +1012,8 @@ __global__ void __launch_bounds__(256) k_obj_boxes(GridD g, int klo, int khi,
   }
   __syncthreads();
   if (tid == 0) {
-    int u[6] = {1 << 30, -(1 << 30), 1 << 30, -(1 << 30), 1 << 30, -(1 << 30)};
     int total = 0;
-    for (int nb = 0; nb < 9; ++nb) {
-      total += ncnt[nb];
-      if (B[nb][0] <= B[nb][1])
-        for (int e = 0; e < 6; ++e) u[e] = (e & 1) ? max(u[e], B[nb][e]) : min(u[e], B[nb][e]);
-    }
-    const bool any = u[0] <= u[1];
-    for (int e = 0; e < 6; ++e) ubox[p * 6 + e] = any ? u[e] : ((e & 1) ? 0 : 1);
+    for (int nb = 0; nb < 9; ++nb) total += ncnt[nb];
     cnt[p] = total;
   }
 }
@@ -1242,7 +1234,8 @@ struct ObjSpread {
                                const double* __restrict__ x, const double* __restrict__ y,
                                const double* __restrict__ z, const double* __restrict__ f1,
                                const double* __restrict__ f2, const double* __restrict__ f3,
-                               const double* __restrict__ area, double (&acc)[OB_CPT][3]) {
+                               const double* __restrict__ area, double (&acc)[OB_CPT][3],
+                               int* __restrict__ err) {
     const int tid = threadIdx.x;
     const ObjItem& it = sm.it;
     const int ncell = it.nx * it.ny * it.nz;
@@ -1281,14 +1274,15 @@ struct ObjSpread {
         window(cc_, lo, hi, w0, a, b);
         // tap w0 has weight 0 in all three components (ws[0] = 0, and wn[0] = 0 unless the
         // centred stencil starts there; the cell-centred form uses ws only), so its terms are
-        // +-0 and leave every sum unchanged: it is left out, which is exact for a finite
-        // force and area (a non-finite one keeps the full window, as in the reference loop)
-        if ((CC || !n0) && a == w0 && fin) ++a;
+        // +-0 and leave every sum unchanged: it is left out.  (Exact for finite forces and
+        // areas; non-finite ones are rejected below.)
+        if ((CC || !n0) && a == w0) ++a;
         w0 -= base_;
         bl = max(a - base_, 0);
         bh = min(b - base_, ext - 1);
       }
       if (d == 0) {
+        if (!fin) atomicOr(err, ERR_FORCE);
 #pragma unroll
         for (int u = 0; u < 4; ++u)
           sm.fw[c][u] = CC ? make_double4(v1 * ws[u], v2 * ws[u], v3 * ws[u], adv)
@@ -1360,7 +1354,7 @@ struct ObjSpread {
                               const double* __restrict__ area,
                               const unsigned* __restrict__ cover, long long ncells,
                               double* __restrict__ o1, double* __restrict__ o2,
-                              double* __restrict__ o3) {
+                              double* __restrict__ o3, int* __restrict__ err) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const ObjItem it = sm.it;
     double acc[OB_CPT][3];
@@ -1401,7 +1395,7 @@ struct ObjSpread {
         npend += total;
         __syncthreads();
         while (npend >= SP_CH) {
-          chunk(sm, SP_CH, g, klo, khi, x, y, z, f1, f2, f3, area, acc);
+          chunk(sm, SP_CH, g, klo, khi, x, y, z, f1, f2, f3, area, acc, err);
           // move the rest (< OB_THREADS entries) to the front
           const int rest = npend - SP_CH;
           const int v = tid < rest ? sm.pend[SP_CH + tid] : 0;
@@ -1412,7 +1406,7 @@ struct ObjSpread {
         }
       }
     }
-    if (npend > 0) chunk(sm, npend, g, klo, khi, x, y, z, f1, f2, f3, area, acc);
+    if (npend > 0) chunk(sm, npend, g, klo, khi, x, y, z, f1, f2, f3, area, acc, err);
     const int ncell = it.nx * it.ny * it.nz;
 #pragma unroll
     for (int r = 0; r < OB_CPT; ++r) {
@@ -1509,7 +1503,7 @@ __global__ void __launch_bounds__(OB_THREADS, MINW)
                  int icap, const int* __restrict__ src, const int* __restrict__ nsrc,
                  const unsigned* __restrict__ cover, long long ncells,
                  int* __restrict__ meta, double* __restrict__ o1, double* __restrict__ o2,
-                 double* __restrict__ o3) {
+                 double* __restrict__ o3, int* __restrict__ err) {
   using O = ObjSpread<CC>;
   __shared__ typename O::Smem sm;
   const int tid = threadIdx.x;
@@ -1547,7 +1541,7 @@ __global__ void __launch_bounds__(OB_THREADS, MINW)
         if (tid < sm.ns) sm.src[tid] = src[(long long)item * OB_NSRC + tid];
         __syncthreads();
         O::item(sm, g, klo, khi, cap_off, x, y, z, f1, f2, f3, area, cover, ncells, o1, o2,
-                o3);
+                o3, err);
       } else {
         items_left = false;
       }
@@ -1564,6 +1558,21 @@ __global__ void __launch_bounds__(OB_THREADS, MINW)
       }
     }
   }
+}
+
+// Flags a non-finite force or area of any point (the tile spread's check; the object spread
+// checks its candidates while preparing them).
+__global__ void __launch_bounds__(256) k_check_forces(long long npts,
+                                                      const double* __restrict__ f1,
+                                                      const double* __restrict__ f2,
+                                                      const double* __restrict__ f3,
+                                                      const double* __restrict__ area,
+                                                      int* __restrict__ err) {
+  bool bad = false;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < npts;
+       q += (long long)gridDim.x * blockDim.x)
+    bad |= !(isfinite(f1[q]) && isfinite(f2[q]) && isfinite(f3[q]) && isfinite(area[q]));
+  if (__ballot(bad) != 0ull && (threadIdx.x & 63) == 0) atomicOr(err, ERR_FORCE);
 }
 
 // Wall halo planes k = 0 and k = k1 of lagr2eulr: zero (the scatter never reaches them and

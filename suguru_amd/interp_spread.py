@@ -17,6 +17,7 @@ that expose `data_ptr()` (torch), in which case nothing leaves HBM.
 from __future__ import annotations
 
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -91,6 +92,9 @@ class IBTransfer:
 
     def close(self):
         if getattr(self, "h", None):
+            # ibx_destroy also destroys the context's live batches: mark them closed
+            for b in list(getattr(self, "_batches", ())):
+                b.h = None
             self.lib.ibx_destroy(self.h)
             self.h = None
 
@@ -193,6 +197,9 @@ class Batch:
         ctx._check(self.lib.ibx_batch_create(ctx.h, ctypes.byref(h), off.size - 1, _ptr(off)))
         self.h = h
         self._keep = None
+        if not hasattr(ctx, "_batches"):
+            ctx._batches = weakref.WeakSet()
+        ctx._batches.add(self)
 
     def bind(self, **bufs):
         b = _lib.IbxBatchBufs(**{k: int(v.data_ptr()) for k, v in bufs.items()})
@@ -252,9 +259,9 @@ class Batch:
         return dict(zip(self.PROFILE_KEYS, list(vals)))
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
             self.lib.ibx_batch_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -256,6 +256,7 @@ def run_experiment(n_train=200_000, n_test=2048, steps=20_000, batch=512, lr=1e-
     losses = s.train(X, Y, steps, batch=batch, lr=lr, seed=seed, log_every=log_every)
     pred = s.predict(Xt)
     err = s.rel_error(pred, Yt)
+    q = lambda v, p: float(torch.quantile(v.double().cpu(), p))
     x, y, z = (Pt[k].reshape(-1).contiguous() for k in range(3))
     _, area = s.geometry(x, y, z)
     ind = s.indicator(pred, Pt.permute(1, 0, 2), area)
@@ -275,12 +276,29 @@ def run_experiment(n_train=200_000, n_test=2048, steps=20_000, batch=512, lr=1e-
         f, _, bad = s.step(x, y, z, tol)
     torch.cuda.synchronize()
     t_sur = (time.perf_counter() - a) / reps
+    # the surrogate path without any fallback (coefficients, MLP, geometry, indicator)
+    a = time.perf_counter()
+    for _ in range(reps):
+        s.step(x, y, z, float("inf"))
+    torch.cuda.synchronize()
+    t_inf = (time.perf_counter() - a) / reps
+    # other tolerances: fallback share, step time, and the largest error that is accepted
+    sweep = []
+    for tl in (3e-3, 1e-2, 3e-2, 1e-1):
+        a = time.perf_counter()
+        for _ in range(reps):
+            _, _, bd = s.step(x, y, z, tl)
+        torch.cuda.synchronize()
+        acc = ~bd
+        sweep.append({"tol": tl, "fallback_fraction": float(bd.double().mean()),
+                      "step_ms": 1e3 * (time.perf_counter() - a) / reps,
+                      "accepted_rel_err_max": float(err[acc].max()) if bool(acc.any()) else 0.0,
+                      "accepted_rel_err_median": q(err[acc], 0.5) if bool(acc.any()) else 0.0})
     fl, _, _ = s.law(x, y, z)
     final = s.rel_error(f.view(3, n_test, -1).permute(1, 0, 2).reshape(n_test, -1),
                         fl.view(3, n_test, -1).permute(1, 0, 2).reshape(n_test, -1))
     if save:
         s.save(save)
-    q = lambda v, p: float(torch.quantile(v.double().cpu(), p))
     return {
         "n_train": n_train, "n_test": n_test, "steps": steps, "batch": batch, "lr": lr,
         "amp_max": amp_max,
@@ -294,5 +312,5 @@ def run_experiment(n_train=200_000, n_test=2048, steps=20_000, batch=512, lr=1e-
         "tol": tol, "fallback_fraction": float(bad.double().mean()),
         "final_rel_err_max": float(final.max()),
         "t_law_ms": 1e3 * t_law, "t_surrogate_step_ms": 1e3 * t_sur,
-        "speedup": t_law / t_sur,
+        "speedup": t_law / t_sur, "t_inference_ms": 1e3 * t_inf, "tol_sweep": sweep,
     }

@@ -178,3 +178,63 @@ def test_batch_replays_follow_moving_points(gpu_ctx_factory):
         for k, a in zip("xyz", (x, y, z)):
             assert np.array_equal(bits(t[k].cpu().numpy()), bits(a)), (step, k)
     b.close()
+
+
+def test_nonfinite_forces_rejected_every_config(gpu_ctx_factory):
+    """A non-finite force or area is an input error (IBX_E_DOMAIN), as a non-finite position
+    is: the reference would spread NaN over its whole window [nint(c-2), nint(c+2)], whose
+    outer taps carry weight 0 and which this build does not visit.  After the error the
+    batch keeps working and gives the oracle's bits again on finite input."""
+    torch = _torch()
+    from suguru_amd import Batch, IbxError
+
+    wl = W.config_small()
+    g, caps = wl.grid, wl.caps
+    ref = [a.reshape(-1, order="F") for a in
+           O.lagr2eulr(g, caps.cap_off, caps.x, caps.y, caps.z, wl.fx, wl.fy, wl.fz, caps.area)]
+    ctx = gpu_ctx_factory(g)
+    t = {k: to_dev(torch, getattr(wl, k)) for k in ("u", "v", "w", "fx", "fy", "fz")}
+    for k in ("x", "y", "z", "area"):
+        t[k] = to_dev(torch, getattr(caps, k))
+    for k in ("ul", "vl", "wl"):
+        t[k] = torch.empty(caps.npts, dtype=torch.float64, device="cuda:0")
+    for k in ("forcex", "forcey", "forcez"):
+        t[k] = torch.empty(g.ncells_halo, dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    b = Batch(ctx, caps.cap_off)
+    b.bind(**t)
+    try:
+        for cfg, name in enumerate(Batch.config_names()):
+            b.set_config(cfg)
+            b.capture(use_graph=False)
+            for key, idx, val in (("fx", 3, np.nan), ("fy", 50, np.inf), ("area", 200, np.nan)):
+                print("case: non-finite", key, "config", name, flush=True)
+                keep = float(t[key][idx])
+                t[key][idx] = val
+                torch.cuda.synchronize()
+                b.run()
+                with pytest.raises(IbxError) as ei:
+                    ctx.sync()
+                assert ei.value.code == -3, (name, key)
+                t[key][idx] = keep
+                torch.cuda.synchronize()
+            b.run()
+            ctx.sync()
+            for k, r in zip(("forcex", "forcey", "forcez"), ref):
+                assert np.array_equal(bits(t[k].cpu().numpy()), bits(r)), (name, k)
+    finally:
+        b.close()
+
+
+def test_context_closed_before_its_batch():
+    """ibx_destroy also destroys the context's live batches; the Python Batch then becomes
+    a no-op to close (no double free, no use of the freed context)."""
+    from suguru_amd import Batch, IBTransfer
+
+    wl = W.config_small(with_fields=False)
+    ctx = IBTransfer(wl.grid, 0)
+    b = Batch(ctx, wl.caps.cap_off)
+    ctx.close()
+    assert b.h is None
+    b.close()
+    del b

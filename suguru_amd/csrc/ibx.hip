@@ -49,7 +49,7 @@ struct TileCfg {
   {TJ, TK, W, NAME, &k_spread<TJ, TK, false>, &k_spread<TJ, TK, true>, false, 0}
 #define IBX_OBJ(W, MINW, NAME) {0, 0, W, NAME, nullptr, nullptr, true, MINW}
 
-const TileCfg kCfgs[] = {
+constexpr TileCfg kCfgs[] = {
     IBX_OBJ(4, 1, "objects-w4"),
     IBX_OBJ(8, 1, "objects-w8"),
     IBX_OBJ(4, 4, "objects-w4-o4"),
@@ -64,6 +64,8 @@ const TileCfg kCfgs[] = {
     IBX_CFG(4, 16, 8, "chunk16x4x16"),
 };
 constexpr int kNumCfgs = (int)(sizeof(kCfgs) / sizeof(kCfgs[0]));
+constexpr int kTileFallback = 5;  // chunk16x8x4: the tile spread used when objects overflow
+static_assert(!kCfgs[kTileFallback].obj, "the fallback must be a tile configuration");
 
 }  // namespace
 
@@ -375,7 +377,9 @@ int collect_errors(ibx_ctx* c) {
     if (h & ERR_NONFINITE) return fail(c, IBX_E_DOMAIN, "non-finite Lagrangian position");
     if (h & ERR_FORCE) return fail(c, IBX_E_DOMAIN, "non-finite Lagrangian force or area");
     if (h & ERR_CAPACITY)
-      return fail(c, IBX_E_CAPACITY, "candidate list capacity exceeded; re-plan the batch");
+      return fail(c, IBX_E_CAPACITY,
+                  "capacity exceeded (tile candidate list, or object work items / sources per "
+                  "item); capture the batch again, or use a tile configuration");
   }
   return IBX_OK;
 }
@@ -587,12 +591,24 @@ static int spread_host(ibx_ctx* c, bool cc, int64_t ncap, const int64_t* cap_off
       (rc = ensure(c, c->s_o1, fb)) || (rc = ensure(c, c->s_o2, fb)) ||
       (rc = ensure(c, c->s_o3, fb)))
     return rc;
-  if ((rc = enqueue_spread(c, c->cfg, cc, ncap, (const long long*)c->s_off.p, npts,
-                           (double*)c->s_x.p, (double*)c->s_y.p, (double*)c->s_z.p,
-                           (double*)c->s_a1.p, (double*)c->s_a2.p, (double*)c->s_a3.p,
-                           (double*)c->s_a4.p, (double*)c->s_o1.p, (double*)c->s_o2.p,
-                           (double*)c->s_o3.p, true)))
-    return rc;
+  // an object configuration that outgrows its item or source capacity (capsules packed more
+  // densely than it plans for) is retried with the tile spread, which sizes its candidate
+  // list here; both give the same bits
+  for (int cfg = c->cfg;;) {
+    if ((rc = enqueue_spread(c, cfg, cc, ncap, (const long long*)c->s_off.p, npts,
+                             (double*)c->s_x.p, (double*)c->s_y.p, (double*)c->s_z.p,
+                             (double*)c->s_a1.p, (double*)c->s_a2.p, (double*)c->s_a3.p,
+                             (double*)c->s_a4.p, (double*)c->s_o1.p, (double*)c->s_o2.p,
+                             (double*)c->s_o3.p, true)))
+      return rc;
+    rc = collect_errors(c);
+    if (rc == IBX_E_CAPACITY && kCfgs[cfg].obj) {
+      cfg = kTileFallback;
+      continue;
+    }
+    if (rc) return rc;
+    break;
+  }
   if ((rc = download(c, o1, c->s_o1, fb)) || (rc = download(c, o2, c->s_o2, fb)) ||
       (rc = download(c, o3, c->s_o3, fb)))
     return rc;
@@ -1005,22 +1021,27 @@ int ibx_batch_autotune(ibx_batch* b, const char* cache_path, int* choice) {
       }
     }
   }
-  int best = 0;
+  int best = -1;
   double best_t = 1e300;
   for (int i = 0; i < kNumCfgs; ++i) {
     b->cfg = i;
     int rc = batch_enqueue(b, true);  // warm-up and sizing
     if (rc) return rc;
-    if ((rc = collect_errors(c))) return rc;
-    b->captured = true;
     double us[IBX_PROF_N];
-    if ((rc = ibx_batch_profile(b, 5, us))) return rc;
-    b->captured = false;
+    if (!(rc = collect_errors(c))) {
+      b->captured = true;
+      rc = ibx_batch_profile(b, 5, us);
+      b->captured = false;
+    }
+    // an object configuration whose item or source capacity this batch exceeds is skipped
+    if (rc == IBX_E_CAPACITY && kCfgs[i].obj) continue;
+    if (rc) return rc;
     if (us[3] < best_t) {
       best_t = us[3];
       best = i;
     }
   }
+  if (best < 0) return fail(c, IBX_E_CAPACITY, "no configuration fits this batch");
   b->cfg = best;
   drop_graph(b);
   if (choice) *choice = best;

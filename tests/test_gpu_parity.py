@@ -209,3 +209,62 @@ def test_full_size_uneven_batch_subset_bitexact_and_identities(gpu_ctx_factory):
     F2 = ctx.lagr2eulr(caps.cap_off, caps.x, caps.y, caps.z, *f, caps.area)
     for a, b, nm in zip(F1, F2, "xyz"):
         assert_bitwise(a, b, "repeat force" + nm)
+
+
+def test_dense_overlapping_capsules_capacity_and_fallback(gpu_ctx_factory):
+    """40 capsules packed into one spot: every work item of the object spread sees 40 source
+    objects, more than its 32.  Object configurations then report IBX_E_CAPACITY, the
+    host entry points retry with the tile spread, and the batch autotuner skips them; every
+    result that is returned is the oracle's, bit for bit."""
+    import torch
+    from suguru_amd import Batch, IbxError
+
+    g = W.make_grid(32, 32, 32, 1.0 / 32)
+    rng = np.random.default_rng(11)
+    ncap, per = 40, 60
+    n = ncap * per
+    c = np.array([15.3, 16.8, 14.1]) * g.dx
+    x = c[0] + rng.uniform(-2.0, 2.0, n) * g.dx
+    y = c[1] + rng.uniform(-2.0, 2.0, n) * g.dy
+    z = c[2] + rng.uniform(-2.0, 2.0, n) * g.dz
+    caps = W.Capsules(np.arange(0, n + 1, per, dtype=np.int64), x, y, z,
+                      rng.uniform(0.5, 1.5, n) * g.dx * g.dy, np.zeros((ncap, 3)), np.ones(ncap),
+                      np.ones(ncap, dtype=np.int64))
+    print("case: 40 overlapping capsules, host entry points", flush=True)
+    u, v, w = W.velocity_field(g, 1)
+    f = W.forces(n, 2)
+    ctx = gpu_ctx_factory(g)
+    run_all(ctx, g, caps, u, v, w, f)
+    # the batch: object configurations overflow, the autotuner picks a tile one
+    dev = "cuda:0"
+    up = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).reshape(-1, order="F"))).to(dev)
+    t = {"u": up(u), "v": up(v), "w": up(w), "fx": up(f[0]), "fy": up(f[1]), "fz": up(f[2])}
+    for k in ("x", "y", "z", "area"):
+        t[k] = up(getattr(caps, k))
+    for k in ("ul", "vl", "wl"):
+        t[k] = torch.empty(n, dtype=torch.float64, device=dev)
+    for k in ("forcex", "forcey", "forcez"):
+        t[k] = torch.empty(g.ncells_halo, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    b = Batch(ctx, caps.cap_off)
+    try:
+        b.bind(**t)
+        names = Batch.config_names()
+        print("case: 40 overlapping capsules, object configuration overflows", flush=True)
+        b.set_config(names.index("objects-w4"))
+        with pytest.raises(IbxError) as ei:  # at the capture's sizing run, or at the sync
+            b.capture(use_graph=False)
+            b.run()
+            ctx.sync()
+        assert ei.value.code == -4
+        print("case: 40 overlapping capsules, autotune", flush=True)
+        choice = b.autotune(None)
+        assert not names[choice].startswith("objects"), names[choice]
+        b.capture(use_graph=True)
+        b.run()
+        ctx.sync()
+        want = O.lagr2eulr(g, caps.cap_off, caps.x, caps.y, caps.z, *f, caps.area)
+        for k, r in zip(("forcex", "forcey", "forcez"), want):
+            assert_bitwise(t[k].cpu().numpy(), r.reshape(-1, order="F"), k)
+    finally:
+        b.close()

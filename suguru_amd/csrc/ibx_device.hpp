@@ -1418,6 +1418,7 @@ struct ObjSpread {
           (long long)i + (long long)(it.y0 + lj) * g.sy + (long long)(it.z0 + lk) * g.sz;
       // the cell and, for i = itot or i = 1, its periodic x-halo duplicate (bound.f90:141-171:
       // var(0) = var(imax), var(i1) = var(1)); the y halos are copied later by k_jhalo
+      // (ordinary stores: partial lines merge in L2 with the rest of their group)
       o1[mm] = acc[r][0], o2[mm] = acc[r][1], o3[mm] = acc[r][2];
       if (i == g.itot)
         o1[mm - g.itot] = acc[r][0], o2[mm - g.itot] = acc[r][1], o3[mm - g.itot] = acc[r][2];
@@ -1478,10 +1479,12 @@ __device__ __forceinline__ void obj_zero_unit(const GridD& g, long long ncells, 
   for (int e = 0; e < OB_ZIT; ++e) {
     const long long m = m0 + 2LL * OB_THREADS * e;
     if (za[e] && zb[e]) {
-      const double2 zz = make_double2(0.0, 0.0);
-      *reinterpret_cast<double2*>(o1 + m) = zz;
-      *reinterpret_cast<double2*>(o2 + m) = zz;
-      *reinterpret_cast<double2*>(o3 + m) = zz;
+      // streaming stores: these lines are not read again by this launch
+      typedef double dv2 __attribute__((ext_vector_type(2)));
+      const dv2 zz = {0.0, 0.0};
+      __builtin_nontemporal_store(zz, reinterpret_cast<dv2*>(o1 + m));
+      __builtin_nontemporal_store(zz, reinterpret_cast<dv2*>(o2 + m));
+      __builtin_nontemporal_store(zz, reinterpret_cast<dv2*>(o3 + m));
     } else {
       if (za[e]) o1[m] = 0.0, o2[m] = 0.0, o3[m] = 0.0;
       if (zb[e]) o1[m + 1] = 0.0, o2[m + 1] = 0.0, o3[m + 1] = 0.0;

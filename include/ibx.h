@@ -177,7 +177,8 @@ int ibx_batch_bind(ibx_batch* b, const ibx_batch_bufs* bufs);
 int ibx_batch_autotune(ibx_batch* b, const char* cache_path, int* choice);
 int ibx_batch_set_config(ibx_batch* b, int choice);
 /* on != 0: run the interpolation on a second stream of the context, concurrently with the
- * spread (they share only read-only inputs); the captured graph then has two branches. */
+ * spread's set-up (they share only read-only inputs); the spread's main kernel starts once
+ * both are done.  The captured graph then has two branches. */
 int ibx_batch_set_overlap(ibx_batch* b, int on);
 int ibx_batch_num_configs(void);
 const char* ibx_batch_config_name(int choice);

@@ -199,7 +199,8 @@ int enqueue_spread_obj(ibx_ctx* c, const TileCfg& cf, bool cc, long long ncap,
                        const long long* d_off, long long npts, const double* x,
                        const double* y, const double* z, const double* f1, const double* f2,
                        const double* f3, const double* area, double* o1, double* o2, double* o3,
-                       hipEvent_t ev_tile, hipEvent_t ev_tile_end, hipEvent_t ev_split) {
+                       hipEvent_t ev_tile, hipEvent_t ev_tile_end, hipEvent_t ev_split,
+                       hipEvent_t ev_before) {
   if (npts >= OB_MAXPTS) return fail(c, IBX_E_ARG, "object spread: more than 2^27 points");
   const int klo = cc ? 0 : 1, khi = cc ? c->grid.ktot + 1 : c->grid.ktot;
   const long long ncells = c->gd.sz * (long long)(c->grid.ktot + 2);
@@ -242,6 +243,9 @@ int enqueue_spread_obj(ibx_ctx* c, const TileCfg& cf, bool cc, long long ncap,
         (const ObjItem*)c->oitems.p, (int)icap, (int*)c->osrc.p, (int*)c->onsrc.p, meta,
         c->d_err);
   }
+  // another stream's work (the interpolation) joins here, before the persistent kernel
+  // takes every compute unit
+  if (ev_before) HIPCHK(c, hipStreamWaitEvent(c->stream, ev_before, 0));
   if (ev_tile) HIPCHK(c, hipEventRecord(ev_tile, c->stream));
   {
     const unsigned nblk = (unsigned)((long long)c->num_cus * cf.wg_per_cu);
@@ -280,11 +284,11 @@ int enqueue_spread(ibx_ctx* c, int cfgid, bool cc, long long ncap, const long lo
                    const double* f1, const double* f2, const double* f3, const double* area,
                    double* o1, double* o2, double* o3, bool size_entries,
                    hipEvent_t ev_tile = nullptr, hipEvent_t ev_split = nullptr,
-                   hipEvent_t ev_tile_end = nullptr) {
+                   hipEvent_t ev_tile_end = nullptr, hipEvent_t ev_before = nullptr) {
   const TileCfg& cf = kCfgs[cfgid];
   if (cf.obj)
     return enqueue_spread_obj(c, cf, cc, ncap, d_off, npts, x, y, z, f1, f2, f3, area, o1, o2,
-                              o3, ev_tile, ev_tile_end, ev_split);
+                              o3, ev_tile, ev_tile_end, ev_split, ev_before);
   TileGeom tg = geom(c->grid, cf, cc);
   int rc;
   if ((rc = ensure(c, c->tcount, (size_t)tg.ntiles * sizeof(unsigned)))) return rc;
@@ -329,6 +333,7 @@ int enqueue_spread(ibx_ctx* c, int cfgid, bool cc, long long ncap, const long lo
     const long long n2 = 2 * c->gd.sz;
     k_zero_kplanes<<<blocks_for((n2 + 1) / 2, 256), 256, 0, c->stream>>>(c->gd, o1, o2, o3);
   }
+  if (ev_before) HIPCHK(c, hipStreamWaitEvent(c->stream, ev_before, 0));
   if (ev_tile) HIPCHK(c, hipEventRecord(ev_tile, c->stream));
   if (!ev_split) {
     fn<<<nblk, SP_THREADS, 0, c->stream>>>(c->gd, tg, off, (int*)c->tmeta.p, 3,
@@ -783,9 +788,10 @@ static int batch_enqueue(ibx_batch* b, bool size_entries, const hipEvent_t* ev =
   }
   rc = enqueue_spread(c, b->cfg, false, b->ncap, (const long long*)b->d_off.p, b->npts, q.x,
                       q.y, q.z, q.fx, q.fy, q.fz, q.area, q.forcex, q.forcey, q.forcez,
-                      size_entries, ev ? ev[2] : nullptr, nullptr, ev ? ev[3] : nullptr);
+                      size_entries, ev ? ev[2] : nullptr, nullptr, ev ? ev[3] : nullptr,
+                      ov ? c->ev_join : nullptr);
   if (rc) return rc;
-  if (ov) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  if (ov) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_join, 0));  // (already joined)
   if (ev) HIPCHK(c, hipEventRecord(ev[4], c->stream));
   return IBX_OK;
 }

@@ -904,6 +904,10 @@ struct ObjGrid {
   __device__ int count() const { return cx * cy * cz; }
 };
 
+constexpr int OB_NBL = 256;  // bins of one capsule's boxes held in LDS
+constexpr int OB_NQL = 512;  // candidate capsules per mask chunk held in LDS
+constexpr int OB_NWC = 64;   // neighbour-mask words per chunk
+
 // Coarse bins of OB_BIN^3 cells over i in [1,itot], j in [1,jtot], k in [0,k1]: bin mask
 // binm[bin][0..nwd) has bit p set when a box of capsule p overlaps the bin.
 constexpr int OB_BIN = 32;
@@ -1107,28 +1111,84 @@ __global__ void __launch_bounds__(256) k_obj_items(GridD g, long long ncap, int 
     for (int r = tid; r < Y * Z; r += 256)
       obj_mark_row(g, cover, B[nb][0], B[nb][1], B[nb][2] + r % Y, B[nb][4] + r / Y);
   }
+  // neighbour mask, in chunks of OB_NWC words: the capsules registered in the bins of p's
+  // boxes (one (bin, word) pair per thread), then one (candidate, image slot) box test per
+  // thread; a bin list or candidate list that would overflow LDS takes the serial path
   const ObjBins bins(g);
-  for (int w = tid; w < nwd; w += 256) {
-    unsigned long long cand = 0ull;
+  __shared__ int blist[OB_NBL];
+  __shared__ int qlist[OB_NQL];
+  __shared__ unsigned long long cand[OB_NWC], res[OB_NWC];
+  __shared__ int nbl, nq;
+  if (tid == 0) {
+    int n = 0;
     for (int nb = 0; nb < 9; ++nb) {
       if (B[nb][0] > B[nb][1]) continue;
       int r[6];
       bins.range(B[nb], r);
       for (int bz = r[4]; bz <= r[5]; ++bz)
         for (int by = r[2]; by <= r[3]; ++by)
-          for (int bx = r[0]; bx <= r[1]; ++bx) cand |= binm[bins.index(bx, by, bz) * nwd + w];
+          for (int bx = r[0]; bx <= r[1]; ++bx) {
+            if (n < OB_NBL) blist[n] = (int)bins.index(bx, by, bz);
+            ++n;
+          }
     }
-    unsigned long long res = 0ull;
-    while (cand) {
-      const int l = __builtin_ctzll(cand);
-      cand &= cand - 1;
-      const int* bq = boxes + ((long long)w * 64 + l) * 54;
-      bool hit = false;
-      for (int n2 = 0; n2 < 9 && !hit; ++n2)
-        for (int n1 = 0; n1 < 9 && !hit; ++n1) hit = boxes_meet(B[n1], bq + n2 * 6);
-      if (hit) res |= 1ull << l;
+    nbl = n;
+  }
+  __syncthreads();
+  for (int w0 = 0; w0 < nwd; w0 += OB_NWC) {
+    const int wn = min(OB_NWC, nwd - w0);
+    for (int t = tid; t < wn; t += 256) cand[t] = 0ull, res[t] = 0ull;
+    if (tid == 0) nq = 0;
+    __syncthreads();
+    if (nbl <= OB_NBL) {
+      for (int pr = tid; pr < nbl * wn; pr += 256) {
+        const unsigned long long v = binm[(long long)blist[pr / wn] * nwd + w0 + pr % wn];
+        if (v) atomicOr(&cand[pr % wn], v);
+      }
+    } else {
+      for (int t = tid; t < wn; t += 256) cand[t] = ~0ull;  // every capsule is a candidate
     }
-    nbm[p * nwd + w] = res;
+    __syncthreads();
+    for (int t = tid; t < wn; t += 256) {
+      unsigned long long m = cand[t];
+      while (m) {
+        const long long q = (long long)(w0 + t) * 64 + __builtin_ctzll(m);
+        m &= m - 1;
+        if (q >= ncap) break;
+        const int k = atomicAdd(&nq, 1);
+        if (k < OB_NQL) qlist[k] = (int)q;
+      }
+    }
+    __syncthreads();
+    if (nq <= OB_NQL) {
+      for (int pr = tid; pr < 9 * nq; pr += 256) {
+        const int q = qlist[pr / 9];
+        const int* bq = boxes + (long long)q * 54 + (pr % 9) * 6;
+        const int v[6] = {bq[0], bq[1], bq[2], bq[3], bq[4], bq[5]};
+        bool hit = false;
+        for (int n1 = 0; n1 < 9 && !hit; ++n1) hit = boxes_meet(B[n1], v);
+        if (hit) atomicOr(&res[q / 64 - w0], 1ull << (q & 63));
+      }
+    } else {
+      for (int t = tid; t < wn; t += 256) {
+        unsigned long long m = cand[t], r = 0ull;
+        while (m) {
+          const int l = __builtin_ctzll(m);
+          m &= m - 1;
+          const long long q = (long long)(w0 + t) * 64 + l;
+          if (q >= ncap) break;
+          const int* bq = boxes + q * 54;
+          bool hit = false;
+          for (int n2 = 0; n2 < 9 && !hit; ++n2)
+            for (int n1 = 0; n1 < 9 && !hit; ++n1) hit = boxes_meet(B[n1], bq + n2 * 6);
+          if (hit) r |= 1ull << l;
+        }
+        res[t] = r;
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < wn; t += 256) nbm[p * nwd + w0 + t] = res[t];
+    __syncthreads();
   }
 }
 

@@ -66,6 +66,18 @@ def declared_symbols(header: str = HEADER):
     return sorted(set(re.findall(r"\b(ibx_[a-z0-9_]+)\s*\(", text)))
 
 
+def torch_runtime_first() -> None:
+    """PyTorch ships its own HIP runtime beside the system one these libraries link.  In one
+    process both work only when PyTorch's is initialised first, so initialise it before the
+    first call into libibx / libmemb (a no-op without PyTorch or without a HIP device)."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
+
+
 def load() -> ctypes.CDLL:
     global _lib
     if _lib is not None:
@@ -74,6 +86,7 @@ def load() -> ctypes.CDLL:
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m suguru_amd.build` "
             "(there is no fallback implementation)")
+    torch_runtime_first()
     lib = ctypes.CDLL(LIB_PATH)
     P = ctypes.c_void_p
     I64 = ctypes.c_int64
@@ -138,6 +151,7 @@ def load_memb() -> ctypes.CDLL:
         raise ImportError(
             f"{MEMB_PATH} is missing: build it with `python -m suguru_amd.build` "
             "(there is no fallback implementation)")
+    torch_runtime_first()
     lib = ctypes.CDLL(MEMB_PATH)
     P = ctypes.c_void_p
     I64 = ctypes.c_int64

@@ -1289,8 +1289,33 @@ struct ObjSpread {
     ObjItem it;
   };
 
+  // The cells of one thread, tid + r OB_THREADS (r = 0 .. OB_CPT-1), as item coordinates
+  // (li, lj, lk), x fastest: the decomposition of tid and of the OB_THREADS step are computed
+  // once per item, and each step is an add with carries (li + sx < 2 nx, lj + sy + 1 < 2 ny),
+  // which keeps integer divisions out of the per-chunk cell loop.
+  struct CellWalk {
+    int nx, ny, li0, lj0, lk0, sx, sy, sz;
+    __device__ explicit CellWalk(const ObjItem& it) {
+      nx = it.nx, ny = it.ny;
+      const int tid = threadIdx.x, nxy = max(nx * ny, 1);
+      li0 = nx ? tid % nx : 0, lj0 = nx && ny ? (tid / nx) % ny : 0, lk0 = tid / nxy;
+      sx = nx ? OB_THREADS % nx : 0, sy = nx && ny ? (OB_THREADS / nx) % ny : 0;
+      sz = OB_THREADS / nxy;
+    }
+    __device__ __forceinline__ void step(int& li, int& lj, int& lk) const {
+      li += sx;
+      const int cx = li >= nx;
+      li -= cx ? nx : 0;
+      lj += sy + cx;
+      const int cy = lj >= ny;
+      lj -= cy ? ny : 0;
+      lk += sz + cy;
+    }
+  };
+
   // one chunk of m <= 64 candidates (sm.pend[0..m)), accumulated into acc
-  __device__ static void chunk(Smem& sm, int m, const GridD& g, int klo, int khi,
+  __device__ static void chunk(Smem& sm, int m, const CellWalk& cw, const GridD& g, int klo,
+                               int khi,
                                const double* __restrict__ x, const double* __restrict__ y,
                                const double* __restrict__ z, const double* __restrict__ f1,
                                const double* __restrict__ f2, const double* __restrict__ f3,
@@ -1372,12 +1397,16 @@ struct ObjSpread {
       }
     }
     __syncthreads();
-    // ordered accumulation: bits ascending = pending order = K order
+    // ordered accumulation: bits ascending = pending order = K order.  The empty asm makes the
+    // walk's start opaque, so the compiler redoes the few adds per chunk instead of holding
+    // all OB_CPT cell coordinates in registers across chunks.
+    int li = cw.li0, lj = cw.lj0, lk = cw.lk0;
+    asm volatile("" : "+v"(li), "+v"(lj), "+v"(lk));
 #pragma unroll
     for (int r = 0; r < OB_CPT; ++r) {
+      if (r) cw.step(li, lj, lk);
       const int cell = tid + r * OB_THREADS;
       if (cell >= ncell) continue;
-      const int li = cell % it.nx, lj = (cell / it.nx) % it.ny, lk = cell / (it.nx * it.ny);
       unsigned long long mk = sm.mx[li] & sm.my[lj] & sm.mz[lk];
       double a1 = acc[r][0], a2 = acc[r][1], a3 = acc[r][2];
       while (mk) {
@@ -1417,6 +1446,7 @@ struct ObjSpread {
                               double* __restrict__ o3, int* __restrict__ err) {
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const ObjItem it = sm.it;
+    const CellWalk cw(it);
     double acc[OB_CPT][3];
 #pragma unroll
     for (int r = 0; r < OB_CPT; ++r) acc[r][0] = acc[r][1] = acc[r][2] = 0.0;
@@ -1455,7 +1485,7 @@ struct ObjSpread {
         npend += total;
         __syncthreads();
         while (npend >= SP_CH) {
-          chunk(sm, SP_CH, g, klo, khi, x, y, z, f1, f2, f3, area, acc, err);
+          chunk(sm, SP_CH, cw, g, klo, khi, x, y, z, f1, f2, f3, area, acc, err);
           // move the rest (< OB_THREADS entries) to the front
           const int rest = npend - SP_CH;
           const int v = tid < rest ? sm.pend[SP_CH + tid] : 0;
@@ -1466,13 +1496,14 @@ struct ObjSpread {
         }
       }
     }
-    if (npend > 0) chunk(sm, npend, g, klo, khi, x, y, z, f1, f2, f3, area, acc, err);
+    if (npend > 0) chunk(sm, npend, cw, g, klo, khi, x, y, z, f1, f2, f3, area, acc, err);
     const int ncell = it.nx * it.ny * it.nz;
+    int li = cw.li0, lj = cw.lj0, lk = cw.lk0;
 #pragma unroll
     for (int r = 0; r < OB_CPT; ++r) {
+      if (r) cw.step(li, lj, lk);
       const int cell = tid + r * OB_THREADS;
       if (cell >= ncell) continue;
-      const int li = cell % it.nx, lj = (cell / it.nx) % it.ny, lk = cell / (it.nx * it.ny);
       const int i = it.x0 + li;
       const long long mm =
           (long long)i + (long long)(it.y0 + lj) * g.sy + (long long)(it.z0 + lk) * g.sz;

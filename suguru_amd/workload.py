@@ -339,5 +339,42 @@ def config_uneven(seed: int = 1234, with_fields: bool = True, ncap: int = 2000) 
     return Workload("batched-512x256x512-2000cap-uneven", g, caps, u, v, w, fx, fy, fz)
 
 
+def lattice_capsules(grid: Grid, radius: float, counts, seed: int, gap_h: float = 1.0):
+    """Centres on a counts[0] x counts[1] x counts[2] lattice (periodic in x and y, wall
+    clearance radius + gap in z), each jittered uniformly by up to half the spare spacing, so
+    every centre distance stays >= 2 radius + gap.  Dense suspensions, where sequential random
+    placement jams."""
+    nx, ny, nz = counts
+    gap = gap_h * grid.dx
+    sx, sy = grid.lx / nx, grid.ly / ny
+    z0, z1 = radius + gap, grid.lz - radius - gap
+    sz = (z1 - z0) / max(nz - 1, 1)
+    jit = 0.5 * (min(sx, sy, sz if nz > 1 else sx) - 2.0 * radius - gap)
+    if jit < 0:
+        raise ValueError("lattice too dense for the radius and gap")
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                d = rng.uniform(-jit, jit, size=3)
+                out.append(((i + 0.5) * sx + d[0], (j + 0.5) * sy + d[1],
+                            min(max(z0 + k * sz + d[2], z0), z1)))
+    return np.asarray(out)
+
+
+def config_dense_share(seed: int = 1234, with_fields: bool = True) -> Workload:
+    """One GPU's share of the dense 1024 x 512 x 256 (itot, jtot, ktot) channel with 8000
+    capsules at 30 % volume fraction: 128 x 512 x 256 cells, 1000 capsules of radius 10.6h,
+    degree 16 (578 points each, 578 000 in all), on a jittered 5 x 20 x 10 lattice."""
+    g = make_grid(128, 512, 256)
+    r = 10.6 * g.dx
+    centers = lattice_capsules(g, r, (5, 20, 10), seed + 11)
+    caps = build_capsules(centers, np.full(len(centers), r), [16] * len(centers))
+    u, v, w = velocity_field(g, seed) if with_fields else (None, None, None)
+    fx, fy, fz = forces(caps.npts, seed + 1)
+    return Workload("share-128x512x256-1000cap-30pct", g, caps, u, v, w, fx, fy, fz)
+
+
 def volume_fraction(grid: Grid, caps: Capsules) -> float:
     return float(np.sum(4.0 / 3.0 * math.pi * caps.radii**3) / (grid.lx * grid.ly * grid.lz))

@@ -62,6 +62,15 @@ def test_plain_moderate_config_bitexact(gpu_ctx_factory):
     run_all(ctx, wl.grid, wl.caps, wl.u, wl.v, wl.w, (wl.fx, wl.fy, wl.fz))
 
 
+def test_dense_share_config_bitexact(gpu_ctx_factory):
+    """One GPU's share of the dense 1024 x 512 x 256 channel: 128 x 512 x 256 cells, 1000
+    capsules of radius 10.6h at 30 % volume fraction (578 000 points), all three transfers."""
+    wl = W.config_dense_share()
+    print("case: dense share 128x512x256, 1000 capsules deg 16", flush=True)
+    ctx = gpu_ctx_factory(wl.grid)
+    run_all(ctx, wl.grid, wl.caps, wl.u, wl.v, wl.w, (wl.fx, wl.fy, wl.fz))
+
+
 def _edge_caps(g, seed):
     """Capsules straddling the periodic edges and corners, touching both walls, one empty,
     uneven degrees; plus points on exact lattice and half-lattice coordinates."""

@@ -17,7 +17,8 @@ capsule-object configurations, k_spread for the tile ones), which writes the who
 field; `path_roofline` is the same figure for the whole interp + spread step.  `surrogate`
 compares the step time with the learned membrane traction against the plain FP64 law
 (`--no-surrogate` skips that leg), and `cpu_baseline` times the CPU oracle on the same
-workload (`--no-cpu` skips it).
+workload (`--no-cpu` skips it).  `capsule_step` times the whole capsule step without the fluid
+update (interpolation, FP64 membrane law, spread, position update; `--no-capsule-step` skips it).
 """
 from __future__ import annotations
 
@@ -113,6 +114,49 @@ def surrogate_leg(transfer_ms):
     return out
 
 
+def capsule_step_leg(torch, steps=20, warmup=3):
+    """The capsule step without the fluid update (suguru_amd.capsule_step): interpolation,
+    FP64 membrane law, spread and position update in the reference's order (main.f90:225-241)
+    on one stream, on the dense share with unit-radius capsules (h = 1/10.6; 1000 capsules of
+    degree 16, 578 000 points), gs = 6250, eb = 0.3 gs, dt = 1e-4.  The velocity field stays
+    frozen: there is no flow solver here.  HIP events between the stages of each timed step."""
+    from suguru_amd import workload as W
+    from suguru_amd.capsule_step import CapsuleStep
+
+    wl = W.config_dense_share(h=1.0 / 10.6)
+    g, caps = wl.grid, wl.caps
+    dev = torch.device("cuda:0")
+
+    def up(a):
+        return torch.from_numpy(np.ascontiguousarray(np.asarray(a).reshape(-1, order="F"))).to(dev)
+
+    u, v, w = up(wl.u), up(wl.v), up(wl.w)
+    x, y, z = up(caps.x), up(caps.y), up(caps.z)
+    F = [torch.empty(g.ncells_halo, dtype=torch.float64, device=dev) for _ in range(3)]
+    torch.cuda.synchronize()
+    st = CapsuleStep(g, caps.cap_off, 16, 6250.0, 0.3 * 6250.0, 1e-4)
+    try:
+        st.bind(u, v, w, x, y, z, *F)
+        for _ in range(warmup):
+            st.step()
+        st.sync()
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(steps)]
+        for k in range(steps):
+            st.step(ev[k])
+        st.sync()
+        stage = np.array([[ev[k][i].elapsed_time(ev[k][i + 1]) for i in range(4)]
+                          for k in range(steps)]) * 1e3
+        total_ms = ev[0][0].elapsed_time(ev[-1][4]) / steps
+    finally:
+        st.close()
+    names = ("interp", "membrane_law", "spread", "update_pos")
+    return {"workload": wl.name + " at h = 1/10.6 (unit-radius capsules)", "capsules": caps.ncap,
+            "points": caps.npts, "steps": steps, "ms_per_step": total_ms,
+            "stages_us_mean": {n: float(stage[:, i].mean()) for i, n in enumerate(names)},
+            "fluid": "not included: frozen velocity field (no flow solver in this build)",
+            "spread_config": "objects-w4 (the one-call default; no autotune)"}
+
+
 def copy_peak_gbs(torch, reps=10):
     """Practical HBM bandwidth: a 1 GiB device-to-device copy (read + write = 2 GiB per copy),
     timed with events on torch's stream."""
@@ -155,6 +199,8 @@ def main():
                     help="interpolation on the same stream as the spread (default: concurrent)")
     ap.add_argument("--profile-only", action="store_true",
                     help="run warmup + steps only (for rocprofv3), no extras")
+    ap.add_argument("--no-capsule-step", action="store_true",
+                    help="skip the capsule-step leg (interp, membrane law, spread, Update_Pos)")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the learned-traction leg (training + step-time comparison)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -325,6 +371,11 @@ def main():
             out["surrogate"] = surrogate_leg(ms)
         except Exception as e:  # reported, never hides the transfer line
             out["surrogate"] = {"error": f"{type(e).__name__}: {e}"}
+    if not args.no_capsule_step:
+        try:
+            out["capsule_step"] = capsule_step_leg(torch)
+        except Exception as e:  # reported, never hides the transfer line
+            out["capsule_step"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

@@ -648,16 +648,11 @@ int ibx_update_pos_dev(ibx_ctx* c, int64_t ncap, const int64_t* cap_off, int64_t
   if ((xold == nullptr) != (yold == nullptr))
     return fail(c, IBX_E_ARG, "xold and yold must both be given or both be NULL");
   HIPCHK(c, hipSetDevice(c->device));
-  int rc = ensure(c, c->s_a4, (size_t)(2 * ncap));
-  if (rc) return rc;
-  signed char* sh = (signed char*)c->s_a4.p;
-  k_advance<<<blocks_for(npts, 256), 256, 0, c->stream>>>(c->gd, c->grid.lz, dt, npts, x, y, z,
-                                                          xold, yold, ul, vl, wl);
-  k_centroid<<<blocks_for(ncap, 64), 64, 0, c->stream>>>(c->gd, ncap, (const long long*)cap_off,
-                                                         x, y, z, ul, vl, wl, centroid, umean,
-                                                         sh);
-  k_shift<<<blocks_for(npts, 256), 256, 0, c->stream>>>(c->gd, ncap, (const long long*)cap_off,
-                                                        npts, sh, x, y);
+  if (npts > 0)
+    k_advance<<<blocks_for(npts, 256), 256, 0, c->stream>>>(c->gd, c->grid.lz, dt, npts, x, y,
+                                                            z, xold, yold, ul, vl, wl);
+  k_centroid<<<(unsigned)ncap, 256, 0, c->stream>>>(c->gd, ncap, (const long long*)cap_off, x,
+                                                    y, z, ul, vl, wl, centroid, umean);
   HIPCHK(c, hipGetLastError());
   return IBX_OK;
 }

@@ -1767,31 +1767,41 @@ __global__ void __launch_bounds__(256) k_advance(GridD g, double lz, double dt, 
   x[q] = xq, y[q] = yq, z[q] = zq;
 }
 
-// one thread per capsule: sequential sums in point order (:135-145), then the shift decision
-__global__ void __launch_bounds__(64) k_centroid(GridD g, long long ncap,
-                                                 const long long* __restrict__ off,
-                                                 const double* __restrict__ x,
-                                                 const double* __restrict__ y,
-                                                 const double* __restrict__ z,
-                                                 const double* __restrict__ ul,
-                                                 const double* __restrict__ vl,
-                                                 const double* __restrict__ wl,
-                                                 double* __restrict__ cen,
-                                                 double* __restrict__ um,
-                                                 signed char* __restrict__ shift) {
-  long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= ncap) return;
-  long long a = off[p], b = off[p + 1];
-  double nl = (double)(b - a);
-  double cx = 0.0, cy = 0.0, cz = 0.0, mu = 0.0, mv = 0.0, mw = 0.0;
-  for (long long q = a; q < b; ++q) {
-    cx = cx + x[q];
-    cy = cy + y[q];
-    cz = cz + z[q];
-    mu = mu + ul[q];
-    mv = mv + vl[q];
-    mw = mw + wl[q];
+// One workgroup per capsule: the 256 threads stage 256 consecutive points at a time in LDS
+// (coalesced loads), and thread c < 6 adds component c of them in point order, so the sums
+// are the reference's sequential ones (:135-145); then the shift decision and the shift.
+__global__ void __launch_bounds__(256) k_centroid(GridD g, long long ncap,
+                                                  const long long* __restrict__ off,
+                                                  double* __restrict__ x,
+                                                  double* __restrict__ y,
+                                                  const double* __restrict__ z,
+                                                  const double* __restrict__ ul,
+                                                  const double* __restrict__ vl,
+                                                  const double* __restrict__ wl,
+                                                  double* __restrict__ cen,
+                                                  double* __restrict__ um) {
+  __shared__ double st[6][256];
+  __shared__ double sum[6];
+  const long long p = blockIdx.x;
+  const int tid = threadIdx.x;
+  const long long a = off[p], b = off[p + 1];
+  double acc = 0.0;  // thread c < 6: component c (x, y, z, u, v, w)
+  for (long long c0 = a; c0 < b; c0 += 256) {
+    const long long q = c0 + tid;
+    if (q < b) {
+      st[0][tid] = x[q], st[1][tid] = y[q], st[2][tid] = z[q];
+      st[3][tid] = ul[q], st[4][tid] = vl[q], st[5][tid] = wl[q];
+    }
+    __syncthreads();
+    const int cnt = (int)(b - c0 < 256 ? b - c0 : 256);
+    if (tid < 6)
+      for (int l = 0; l < cnt; ++l) acc = acc + st[tid][l];
+    __syncthreads();
   }
+  if (tid < 6) sum[tid] = acc;
+  __syncthreads();
+  double nl = (double)(b - a);
+  double cx = sum[0], cy = sum[1], cz = sum[2], mu = sum[3], mv = sum[4], mw = sum[5];
   cx = cx / nl, cy = cy / nl, cz = cz / nl;
   mu = mu / nl, mv = mv / nl, mw = mw / nl;
   int sx = 0, sy = 0;
@@ -1811,32 +1821,20 @@ __global__ void __launch_bounds__(64) k_centroid(GridD g, long long ncap,
     cy = cy + g.ly;
     sy = (sy == -1) ? 2 : 1;
   }
-  cen[3 * p] = cx, cen[3 * p + 1] = cy, cen[3 * p + 2] = cz;
-  um[3 * p] = mu, um[3 * p + 1] = mv, um[3 * p + 2] = mw;
-  shift[2 * p] = (signed char)sx;
-  shift[2 * p + 1] = (signed char)sy;
-}
-
-__global__ void __launch_bounds__(256) k_shift(GridD g, long long ncap,
-                                               const long long* __restrict__ off,
-                                               long long npts, const signed char* __restrict__ shift,
-                                               double* __restrict__ x, double* __restrict__ y) {
-  long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= npts) return;
-  long long lo = 0, hi = ncap - 1;
-  while (lo < hi) {
-    long long mid = (lo + hi + 1) >> 1;
-    if (off[mid] <= q)
-      lo = mid;
-    else
-      hi = mid - 1;
+  if (tid == 0) {
+    cen[3 * p] = cx, cen[3 * p + 1] = cy, cen[3 * p + 2] = cz;
+    um[3 * p] = mu, um[3 * p + 1] = mv, um[3 * p + 2] = mw;
   }
-  int sx = shift[2 * lo], sy = shift[2 * lo + 1];
-  // same per-point operation sequence as the reference's two successive loops
-  if (sx == -1 || sx == 2) x[q] = x[q] - g.lx;
-  if (sx == 1 || sx == 2) x[q] = x[q] + g.lx;
-  if (sy == -1 || sy == 2) y[q] = y[q] - g.ly;
-  if (sy == 1 || sy == 2) y[q] = y[q] + g.ly;
+  // the periodic shift of the whole capsule (:147-186), by the workgroup that owns its points
+  // (all loads above precede these stores); per point the same operation sequence as the
+  // reference's two successive loops
+  if (sx == 0 && sy == 0) return;  // uniform
+  for (long long q = a + tid; q < b; q += 256) {
+    if (sx == -1 || sx == 2) x[q] = x[q] - g.lx;
+    if (sx == 1 || sx == 2) x[q] = x[q] + g.lx;
+    if (sy == -1 || sy == 2) y[q] = y[q] - g.ly;
+    if (sy == 1 || sy == 2) y[q] = y[q] + g.ly;
+  }
 }
 
 }  // namespace ibx

@@ -363,11 +363,13 @@ def lattice_capsules(grid: Grid, radius: float, counts, seed: int, gap_h: float 
     return np.asarray(out)
 
 
-def config_dense_share(seed: int = 1234, with_fields: bool = True) -> Workload:
+def config_dense_share(seed: int = 1234, with_fields: bool = True,
+                       h: float = H_DEFAULT) -> Workload:
     """One GPU's share of the dense 1024 x 512 x 256 (itot, jtot, ktot) channel with 8000
     capsules at 30 % volume fraction: 128 x 512 x 256 cells, 1000 capsules of radius 10.6h,
-    degree 16 (578 points each, 578 000 in all), on a jittered 5 x 20 x 10 lattice."""
-    g = make_grid(128, 512, 256)
+    degree 16 (578 points each, 578 000 in all), on a jittered 5 x 20 x 10 lattice.  With
+    h = 1/10.6 the capsules have unit radius, the membrane law's stress-free size."""
+    g = make_grid(128, 512, 256, h)
     r = 10.6 * g.dx
     centers = lattice_capsules(g, r, (5, 20, 10), seed + 11)
     caps = build_capsules(centers, np.full(len(centers), r), [16] * len(centers))

@@ -370,8 +370,10 @@ __device__ void law(const Geom& o, const double* __restrict__ R, double gs, doub
 // grid, no load).  Persistent: workgroup b handles capsules b, b + gridDim.x, ...
 // LDS: coarse coefficients [3][N][N] a and b, then a work region reused by the phases.
 // scratch: per workgroup, longitude DFT of the 2x-grid outputs [nt][6][N][2].
+// Registers: capped for two wavefronts per SIMD (256 VGPRs, about 40 of them spilled).  With
+// no cap the law takes 256 VGPRs + 40 AGPRs, one wavefront per SIMD, and runs 1.6x slower.
 template <int MODE>
-__global__ void __launch_bounds__(THREADS)
+__global__ void __launch_bounds__(THREADS, 2)
     k_mem(Tables T, long long ncap, double gs, double eb, const double* __restrict__ x,
           const double* __restrict__ y, const double* __restrict__ z, double* __restrict__ o0,
           double* __restrict__ o1, double* __restrict__ o2, double* __restrict__ o3,

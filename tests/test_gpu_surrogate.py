@@ -86,7 +86,7 @@ def test_indicator_separates_law_and_network(trained):
 
 def test_inference_is_faster_than_the_law(trained):
     """The surrogate path without fallback (coefficients, MLP, geometry, indicator) against
-    the FP64 law on the same 256 capsules (a generous bar; the bench measures 3.7x at 2048)."""
+    the FP64 law on the same 256 capsules (a generous bar; the bench measures 2.6x at 2048)."""
     torch = _torch()
     s, _, _, Pt, _, _, _ = trained
     x, y, z = (Pt[k].reshape(-1).contiguous() for k in range(3))

@@ -88,6 +88,11 @@ const char* ibx_last_error(const ibx_ctx* ctx);
 int ibx_set_stream(ibx_ctx* ctx, void* hip_stream);
 /* Wait for the context's stream; reports errors raised by device-side checks. */
 int ibx_sync(ibx_ctx* ctx);
+/* Spread configuration of the one-call entry points (ibx_lagr2eulr, ibx_lagr2eulrind,
+ * ibx_spread_f64_dev, ibx_spread_cc_f64_dev): an index of ibx_batch_config_name's list.
+ * Default: objects-w8-o4.  A batch chooses its own (ibx_batch_set_config / _autotune).  No
+ * reference counterpart: the reference has one loop nest. */
+int ibx_set_config(ibx_ctx* ctx, int choice);
 /* Library version string. */
 const char* ibx_version(void);
 

@@ -111,6 +111,7 @@ def load() -> ctypes.CDLL:
         "ibx_batch_bind": (ctypes.c_int, [P, ctypes.POINTER(IbxBatchBufs)]),
         "ibx_batch_autotune": (ctypes.c_int, [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
         "ibx_batch_set_config": (ctypes.c_int, [P, ctypes.c_int]),
+        "ibx_set_config": (ctypes.c_int, [P, ctypes.c_int]),
         "ibx_batch_set_overlap": (ctypes.c_int, [P, ctypes.c_int]),
         "ibx_batch_num_configs": (ctypes.c_int, []),
         "ibx_batch_config_name": (ctypes.c_char_p, [ctypes.c_int]),

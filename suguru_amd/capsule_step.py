@@ -32,10 +32,11 @@ class CapsuleStep:
 
     gs, eb: shear and bending moduli of the law (gscell, ebend; param.f90:88,91).
     dt: the time step of the position update (main.f90:119).
+    spread_config: the spread's configuration (index or name; default: the library's).
     """
 
     def __init__(self, grid, cap_off, degree: int, gs: float, eb: float, dt: float,
-                 device: int = 0):
+                 device: int = 0, spread_config=None):
         import torch
 
         off = _offsets(cap_off)
@@ -46,6 +47,8 @@ class CapsuleStep:
             self.plan.close()
             raise ValueError(f"every capsule must have the {n} points of degree {degree}")
         self.ctx = IBTransfer(grid, device)
+        if spread_config is not None:
+            self.ctx.set_config(spread_config)
         self.grid, self.ncap, self.npts = grid, ncap, ncap * n
         self.gs, self.eb, self.dt = float(gs), float(eb), float(dt)
         self.dev = torch.device("cuda", device)

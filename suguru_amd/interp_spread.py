@@ -114,6 +114,13 @@ class IBTransfer:
     def sync(self):
         self._check(self.lib.ibx_sync(self.h))
 
+    def set_config(self, choice):
+        """Spread configuration of the one-call entry points: an index or a name of
+        Batch.config_names() (default objects-w8-o4)."""
+        if isinstance(choice, str):
+            choice = Batch.config_names().index(choice)
+        self._check(self.lib.ibx_set_config(self.h, int(choice)))
+
     # ---- host arrays -------------------------------------------------------------------
     def eulr2lagr(self, u, v, w, cap_off, x, y, z):
         g = self.grid

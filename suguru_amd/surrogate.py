@@ -211,12 +211,19 @@ class Surrogate:
         bad = ind > tol
         f = pred.double().view(ncap, 3, self.npts).permute(1, 0, 2).reshape(3, -1).contiguous()
         nb = int(bad.sum().item())
-        if nb:
+        if 2 * nb >= ncap:
+            # most capsules fall back: the law on the whole batch and a per-capsule select.
+            # The law runs one workgroup per capsule, so a capsule's traction does not depend
+            # on the batch it is computed in, and this equals the gathered path bit for bit.
+            fl, _, _ = self.law(x, y, z)
+            sel = bad.view(1, ncap, 1)
+            f = t.where(sel, fl.view(3, ncap, self.npts), f.view(3, ncap, self.npts))
+            f = f.reshape(3, -1).contiguous()
+        elif nb:
             idx = t.nonzero(bad).view(-1)
-            sub = [a.view(ncap, -1)[idx].reshape(-1).contiguous() for a in (x, y, z)]
+            sub = [a.view(ncap, -1).index_select(0, idx).reshape(-1) for a in (x, y, z)]
             fl, _, _ = self.law(*sub)
-            fv = f.view(3, ncap, self.npts)
-            fv[:, idx] = fl.view(3, nb, self.npts)
+            f.view(3, ncap, self.npts).index_copy_(1, idx, fl.view(3, nb, self.npts))
         return f, area, bad
 
     # ---- persistence ---------------------------------------------------------------------------
@@ -285,6 +292,8 @@ def run_experiment(n_train=200_000, n_test=2048, steps=20_000, batch=512, lr=1e-
     # other tolerances: fallback share, step time, and the largest error that is accepted
     sweep = []
     for tl in (3e-3, 1e-2, 3e-2, 1e-1):
+        s.step(x, y, z, tl)  # untimed, like the warm-up of the timings above
+        torch.cuda.synchronize()
         a = time.perf_counter()
         for _ in range(reps):
             _, _, bd = s.step(x, y, z, tl)

@@ -107,3 +107,28 @@ def test_inference_is_faster_than_the_law(trained):
     t_sur = (time.perf_counter() - t0) / reps
     print(f"law {1e3 * t_law:.3f} ms, surrogate path {1e3 * t_sur:.3f} ms", flush=True)
     assert t_sur < t_law
+
+
+@pytest.mark.parametrize("q", [0.25, 0.75])
+def test_partial_fallback_selects_per_capsule(trained, q):
+    """A tolerance between the indicator's quantiles sends part of the batch to the FP64 law
+    (q = 0.25: three quarters fall back, the whole-batch-law branch; q = 0.75: a quarter, the
+    gathered branch).  Each capsule then carries the law's traction bit for bit if it fell
+    back, and the network's prediction otherwise."""
+    torch = _torch()
+    s, Xt, _, Pt, _, _, _ = trained
+    x, y, z = (Pt[k].reshape(-1).contiguous() for k in range(3))
+    n = Xt.shape[0]
+    pred = s.predict(s.coeffs(x, y, z).float())
+    _, area = s.geometry(x, y, z)
+    ind = s.indicator(pred, Pt.permute(1, 0, 2), area)
+    tol = float(torch.quantile(ind.double(), q))
+    f, _, bad = s.step(x, y, z, tol)
+    nb = int(bad.sum())
+    print(f"q={q}: {nb} of {n} capsules fall back", flush=True)
+    assert 0 < nb < n
+    f_law, _, _ = s.law(x, y, z)
+    fp = pred.double().view(n, 3, -1).permute(1, 0, 2)
+    fv, lv = f.view(3, n, -1), f_law.view(3, n, -1)
+    assert torch.equal(fv[:, bad], lv[:, bad])
+    assert torch.equal(fv[:, ~bad], fp[:, ~bad])

@@ -78,13 +78,14 @@ class CapsuleStep:
         if not stream:
             raise ValueError("a non-null stream handle is needed (0 selects each library's own "
                              "stream, which are not ordered with each other)")
-        for name, t, k in (("u", u, None), ("v", v, None), ("w", w, None), ("x", x, self.npts),
-                           ("y", y, self.npts), ("z", z, self.npts), ("forcex", forcex, None),
-                           ("forcey", forcey, None), ("forcez", forcez, None)):
+        nf = self.grid.ncells_halo
+        for name, t, k in (("u", u, nf), ("v", v, nf), ("w", w, nf), ("x", x, self.npts),
+                           ("y", y, self.npts), ("z", z, self.npts), ("forcex", forcex, nf),
+                           ("forcey", forcey, nf), ("forcez", forcez, nf)):
             if t.dtype != torch.float64 or t.device != self.dev or not t.is_contiguous():
                 raise ValueError(f"{name}: contiguous float64 on {self.dev} expected")
-            if k is not None and t.numel() != k:
-                raise ValueError(f"{name}: {k} values expected")
+            if t.numel() != k:  # the libraries see raw pointers: sizes are checked here
+                raise ValueError(f"{name}: {k} values expected, got {t.numel()}")
         self.ctx.set_stream(stream)
         self.plan.set_stream(stream)
         self.stream = torch.cuda.ExternalStream(stream, device=self.dev)

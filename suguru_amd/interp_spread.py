@@ -77,6 +77,20 @@ def _offsets(cap_off):
     return off
 
 
+def _dev_check(name, t, n, integer=False):
+    """A tensor that the library reads or writes through a raw pointer: its size, dtype,
+    contiguity and placement are checked here, because the library cannot see them."""
+    import torch
+
+    want = torch.int64 if integer else torch.float64
+    if not isinstance(t, torch.Tensor) or t.dtype != want or not t.is_cuda:
+        raise ValueError(f"{name}: a {want} tensor on the GPU expected")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: contiguous tensor expected")
+    if t.numel() != n:
+        raise ValueError(f"{name}: {n} values expected, got {t.numel()}")
+
+
 class IBTransfer:
     """One libibx context: a grid on one device."""
 
@@ -173,12 +187,25 @@ class IBTransfer:
     # ---- HBM-resident tensors (torch) ----------------------------------------------------
     def interp_dev(self, u, v, w, cap_off, x, y, z, ul, vl, wl):
         n = int(x.numel())
+        nf = self.grid.ncells_halo
+        for nm, t in (("u", u), ("v", v), ("w", w)):
+            _dev_check(nm, t, nf)
+        for nm, t in (("x", x), ("y", y), ("z", z), ("ul", ul), ("vl", vl), ("wl", wl)):
+            _dev_check(nm, t, n)
+        _dev_check("cap_off", cap_off, int(cap_off.numel()), integer=True)
         self._check(self.lib.ibx_interp_f64_dev(self.h, _ptr(u), _ptr(v), _ptr(w),
                                                 int(cap_off.numel()) - 1, _ptr(cap_off), n,
                                                 _ptr(x), _ptr(y), _ptr(z), _ptr(ul), _ptr(vl),
                                                 _ptr(wl)))
 
     def spread_dev(self, cap_off, x, y, z, fx, fy, fz, area, forcex, forcey, forcez, cc=False):
+        n, nf = int(x.numel()), self.grid.ncells_halo
+        for nm, t in (("x", x), ("y", y), ("z", z), ("fx", fx), ("fy", fy), ("fz", fz),
+                      ("area", area)):
+            _dev_check(nm, t, n)
+        for nm, t in (("forcex", forcex), ("forcey", forcey), ("forcez", forcez)):
+            _dev_check(nm, t, nf)
+        _dev_check("cap_off", cap_off, int(cap_off.numel()), integer=True)
         fn = self.lib.ibx_spread_cc_f64_dev if cc else self.lib.ibx_spread_f64_dev
         self._check(fn(self.h, int(cap_off.numel()) - 1, _ptr(cap_off), int(x.numel()), _ptr(x),
                        _ptr(y), _ptr(z), _ptr(fx), _ptr(fy), _ptr(fz), _ptr(area),
@@ -186,6 +213,13 @@ class IBTransfer:
 
     def update_pos_dev(self, cap_off, x, y, z, ul, vl, wl, dt, centroid, umean, xold=None,
                        yold=None):
+        n, ncap = int(x.numel()), int(cap_off.numel()) - 1
+        for nm, t in (("x", x), ("y", y), ("z", z), ("ul", ul), ("vl", vl), ("wl", wl)) + (
+                (("xold", xold), ("yold", yold)) if xold is not None else ()):
+            _dev_check(nm, t, n)
+        for nm, t in (("centroid", centroid), ("umean", umean)):
+            _dev_check(nm, t, 3 * ncap)
+        _dev_check("cap_off", cap_off, ncap + 1, integer=True)
         self._check(self.lib.ibx_update_pos_dev(self.h, int(cap_off.numel()) - 1, _ptr(cap_off),
                                                 int(x.numel()), float(dt), _ptr(x), _ptr(y),
                                                 _ptr(z), _ptr(xold), _ptr(yold), _ptr(ul),
@@ -203,12 +237,16 @@ class Batch:
         h = ctypes.c_void_p()
         ctx._check(self.lib.ibx_batch_create(ctx.h, ctypes.byref(h), off.size - 1, _ptr(off)))
         self.h = h
+        self.npts = int(off[-1])
         self._keep = None
         if not hasattr(ctx, "_batches"):
             ctx._batches = weakref.WeakSet()
         ctx._batches.add(self)
 
     def bind(self, **bufs):
+        nf, n = self.ctx.grid.ncells_halo, self.npts
+        for k, v in bufs.items():
+            _dev_check(k, v, nf if k in ("u", "v", "w", "forcex", "forcey", "forcez") else n)
         b = _lib.IbxBatchBufs(**{k: int(v.data_ptr()) for k, v in bufs.items()})
         self._keep = bufs
         self.ctx._check(self.lib.ibx_batch_bind(self.h, ctypes.byref(b)))

@@ -292,3 +292,42 @@ def test_dense_share_every_config_bitexact(gpu_ctx_factory):
     finally:
         b.close()
     assert not failures, failures[:6]
+
+
+def test_device_wrappers_reject_wrong_sizes(gpu_ctx_factory):
+    """The device entry points take raw pointers; the Python wrappers check every tensor's
+    size, dtype and placement before the call (a short field would otherwise be read out of
+    bounds), and nothing reaches the library."""
+    torch = _torch()
+    from suguru_amd import Batch
+
+    wl = W.config_small(with_fields=False)
+    g, caps = wl.grid, wl.caps
+    ctx = gpu_ctx_factory(g)
+    e = lambda n, dt=torch.float64: torch.zeros(n, dtype=dt, device="cuda:0")
+    n, nf = caps.npts, g.ncells_halo
+    off = torch.from_numpy(caps.cap_off).to("cuda:0")
+    ok = dict(u=e(nf), v=e(nf), w=e(nf), x=e(n), y=e(n), z=e(n), ul=e(n), vl=e(n), wl=e(n))
+    with pytest.raises(ValueError, match="u:"):
+        ctx.interp_dev(e(nf - 1), ok["v"], ok["w"], off, ok["x"], ok["y"], ok["z"], ok["ul"],
+                       ok["vl"], ok["wl"])
+    with pytest.raises(ValueError, match="forcez:"):
+        ctx.spread_dev(off, ok["x"], ok["y"], ok["z"], e(n), e(n), e(n), e(n), e(nf), e(nf),
+                       e(nf // 2))
+    with pytest.raises(ValueError, match="area:"):
+        ctx.spread_dev(off, ok["x"], ok["y"], ok["z"], e(n), e(n), e(n), e(n, torch.float32),
+                       e(nf), e(nf), e(nf))
+    with pytest.raises(ValueError, match="centroid:"):
+        ctx.update_pos_dev(off, ok["x"], ok["y"], ok["z"], ok["ul"], ok["vl"], ok["wl"], 1e-3,
+                           e(caps.ncap), e(3 * caps.ncap))
+    with pytest.raises(ValueError, match="x:"):
+        ctx.interp_dev(ok["u"], ok["v"], ok["w"], off, ok["x"].cpu(), ok["y"], ok["z"],
+                       ok["ul"], ok["vl"], ok["wl"])
+    b = Batch(ctx, caps.cap_off)
+    try:
+        bufs = dict(ok, fx=e(n), fy=e(n), fz=e(n), area=e(n), forcex=e(nf), forcey=e(nf),
+                    forcez=e(nf - 16))
+        with pytest.raises(ValueError, match="forcez:"):
+            b.bind(**bufs)
+    finally:
+        b.close()

@@ -1,9 +1,10 @@
 """Benchmark of the immersed-boundary transfer hot path on one MI355X.
 
 One step = one interpolation (eulr2lagr) plus one spread (lagr2eulr) over the whole batch,
-with inputs resident in HBM.  The step is replayed as a captured hipGraph (reported as
-graph_ms_per_step), then timed as K eager launches with HIP events between its stages, which
-give `value` and the per-kernel durations of the roofline.  Default workload: the
+with inputs resident in HBM.  The step is captured as one hipGraph; K replays between two
+synchronisations give `value` and `ms_per_step`.  The same step is then launched eagerly K
+times with HIP events between its stages (`eager_ms_per_step`); those events give the
+per-kernel durations of the roofline.  Default workload: the
 512 x 256 x 512 channel (h = 1/64, x,z periodic, y walls) with 2000 seeded non-overlapping
 capsules of radius U(4h,10h) and degree 8..24 by radius (1.16 M Lagrangian points).
 `--workload dense`: one GPU's share of the dense 1024 x 512 x 256 channel, 128 x 512 x 256 cells
@@ -293,7 +294,7 @@ def main():
     ms = 1000.0 * elapsed / args.steps
     slots = batch.slot_times()
     if args.profile_only:
-        out = {"ms_per_step": ms, "graph_ms_per_step": graph_ms,
+        out = {"ms_per_step": graph_ms, "eager_ms_per_step": ms, "graph_ms_per_step": graph_ms,
                "config": Batch.config_names()[choice],
                "tile_spread_us": float(np.mean([s["tile_spread_us"] for s in slots]))}
         print(json.dumps(out))
@@ -311,14 +312,18 @@ def main():
     practical = copy_peak_gbs(torch)
     out = {
         "metric": "IB interp+spread Lag-pts/s & HBM GB/s; step time surrogate vs plain @1e-3",
-        "value": n / (ms * 1e-3),
+        "value": n / (graph_ms * 1e-3),
         "unit": "Lag-pts/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": graph_ms,
         "graph_ms_per_step": graph_ms,
         "graph_ms_per_step_median": float(np.median(replay_ms)),
+        "eager_ms_per_step": ms,
+        "timing": ("value and ms_per_step: K graph replays between two synchronisations; "
+                   "eager_ms_per_step: K eager launches with HIP events between the stages, "
+                   "which give the roofline's kernel durations"),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -354,10 +359,10 @@ def main():
             "timing": "HIP events on the batch stream around the kernel in each timed step",
         },
         "path_roofline": {
-            "achieved": (b_interp + b_spread) / (ms * 1e-3) / 1e9,
+            "achieved": (b_interp + b_spread) / (graph_ms * 1e-3) / 1e9,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": (b_interp + b_spread) / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "frac": (b_interp + b_spread) / (graph_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
             "algorithmic_bytes_interp": b_interp,
             "algorithmic_bytes_spread": b_spread,
             "support_cells_U": U,
@@ -368,7 +373,7 @@ def main():
     }
     if not args.no_surrogate:
         try:
-            out["surrogate"] = surrogate_leg(ms)
+            out["surrogate"] = surrogate_leg(graph_ms)
         except Exception as e:  # reported, never hides the transfer line
             out["surrogate"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_capsule_step:

@@ -209,8 +209,9 @@ def main():
                          "for scaling rocprofv3's FETCH_SIZE and WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU only; the path's multi-GPU sharding is "
-              "described in DESIGN.md and is not run.", file=sys.stderr)
+        print("bench.py: measures one GPU only; the path's sharding over several GPUs (slabs in "
+              "x and y with a ghost-capsule exchange) is a design and is not run.",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

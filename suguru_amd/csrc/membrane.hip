@@ -28,7 +28,7 @@ namespace memb {
 
 constexpr int NQ = 15;        // derivatives of total order <= 4
 constexpr int RING = 3;       // fine latitude rings processed together
-constexpr int THREADS = 256;
+constexpr int THREADS = 128;     // per capsule workgroup (two wavefronts)
 constexpr int NREF = 56;      // reference-state doubles per fine node
 constexpr int LMAX = 23;
 
@@ -372,6 +372,9 @@ __device__ void law(const Geom& o, const double* __restrict__ R, double gs, doub
 // scratch: per workgroup, longitude DFT of the 2x-grid outputs [nt][6][N][2].
 // Registers: capped for two wavefronts per SIMD (256 VGPRs, about 40 of them spilled).  With
 // no cap the law takes 256 VGPRs + 40 AGPRs, one wavefront per SIMD, and runs 1.6x slower.
+// 128 threads per capsule and four capsules per CU beat 256 threads and two by 11 %: the
+// per-node phase takes RING x np nodes at a time (3 x 68 = 204 at L = 16), which one round of
+// 256 threads covers no better than two rounds of 128.
 template <int MODE>
 __global__ void __launch_bounds__(THREADS, 2)
     k_mem(Tables T, long long ncap, double gs, double eb, const double* __restrict__ x,
@@ -789,7 +792,7 @@ int memb_plan_create(memb_plan** out, int L, int device) {
   p->lds = lds_bytes(T);
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-  p->num_wg = 2 * cus;
+  p->num_wg = 4 * cus;  // four capsule workgroups per CU (LDS and the register cap allow it)
   // per workgroup: longitude DFT of the 2x-grid outputs, and the force-grid basis vectors
   p->scratch_wg = std::max<size_t>((size_t)nt * 6 * N * 2, (size_t)N * np2 * 6);
   void* s = nullptr;

@@ -46,65 +46,74 @@ enum Epi {
 
 // A: if !TA, A[m][k] = Ag[m * lda + k]; if TA, A[m][k] = Ag[k * lda + m]
 // B: if TB,  B[k][n] = Bg[n * ldb + k]; if !TB, B[k][n] = Bg[k * ldb + n]
-template <bool TA, bool TB, int EPI>
+// Tile BM x BN (64 x 64, or 32 x 32 when the 64-tile grid would leave compute units idle):
+// four wavefronts in a 2 x 2 arrangement, each owning (BM/2) x (BN/2) as 16 x 16 MFMA blocks.
+// Each output element sums its K-slices in the same order whatever the tile, so both tiles
+// give the same bits.
+template <int BM, int BN, bool TA, bool TB, int EPI>
 __global__ void __launch_bounds__(THREADS)
     k_gemm(int M, int N, int K, const __bf16* __restrict__ Ag, int lda,
            const __bf16* __restrict__ Bg, int ldb, const float* __restrict__ bias,
            const float* __restrict__ aux, float* __restrict__ out0, void* __restrict__ out1,
            int ldc) {
-  __shared__ __attribute__((aligned(16))) __bf16 As[TM * LDK];
-  __shared__ __attribute__((aligned(16))) __bf16 Bs[TN * LDK];
+  constexpr int FM = BM / 32, FN = BN / 32;  // MFMA blocks per wavefront
+  __shared__ __attribute__((aligned(16))) __bf16 As[BM * LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[BN * LDK];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
-  const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
-  f32x4 acc[2][2];
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int wm = (wv >> 1) * (BM / 2), wn = (wv & 1) * (BN / 2);
+  f32x4 acc[FM][FN];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < FN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   for (int k0 = 0; k0 < K; k0 += TK) {
-    // stage A (64 x 32) and B (32 x 64 as [n][k]): 256 threads x 8 bf16 each
-    if (!TA) {
-      const int r = tid >> 2, kc = (tid & 3) * 8;
-      *reinterpret_cast<bf16x8*>(&As[r * LDK + kc]) =
-          *reinterpret_cast<const bf16x8*>(&Ag[(size_t)(m0 + r) * lda + k0 + kc]);
-    } else {
-      const int kk = tid >> 3, mc = (tid & 7) * 8;
-      const bf16x8 v = *reinterpret_cast<const bf16x8*>(&Ag[(size_t)(k0 + kk) * lda + m0 + mc]);
+    // stage A (BM x 32) and B (32 x BN as [n][k]): 8 bf16 per thread, BM * 4 (BN * 4) threads
+    if (tid < BM * 4) {
+      if (!TA) {
+        const int r = tid >> 2, kc = (tid & 3) * 8;
+        *reinterpret_cast<bf16x8*>(&As[r * LDK + kc]) =
+            *reinterpret_cast<const bf16x8*>(&Ag[(size_t)(m0 + r) * lda + k0 + kc]);
+      } else {
+        const int kk = tid / (BM / 8), mc = (tid % (BM / 8)) * 8;
+        const bf16x8 v = *reinterpret_cast<const bf16x8*>(&Ag[(size_t)(k0 + kk) * lda + m0 + mc]);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) As[(mc + e) * LDK + kk] = v[e];
+        for (int e = 0; e < 8; ++e) As[(mc + e) * LDK + kk] = v[e];
+      }
     }
-    if (TB) {
-      const int r = tid >> 2, kc = (tid & 3) * 8;
-      *reinterpret_cast<bf16x8*>(&Bs[r * LDK + kc]) =
-          *reinterpret_cast<const bf16x8*>(&Bg[(size_t)(n0 + r) * ldb + k0 + kc]);
-    } else {
-      const int kk = tid >> 3, nc = (tid & 7) * 8;
-      const bf16x8 v = *reinterpret_cast<const bf16x8*>(&Bg[(size_t)(k0 + kk) * ldb + n0 + nc]);
+    if (tid < BN * 4) {
+      if (TB) {
+        const int r = tid >> 2, kc = (tid & 3) * 8;
+        *reinterpret_cast<bf16x8*>(&Bs[r * LDK + kc]) =
+            *reinterpret_cast<const bf16x8*>(&Bg[(size_t)(n0 + r) * ldb + k0 + kc]);
+      } else {
+        const int kk = tid / (BN / 8), nc = (tid % (BN / 8)) * 8;
+        const bf16x8 v = *reinterpret_cast<const bf16x8*>(&Bg[(size_t)(k0 + kk) * ldb + n0 + nc]);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) Bs[(nc + e) * LDK + kk] = v[e];
+        for (int e = 0; e < 8; ++e) Bs[(nc + e) * LDK + kk] = v[e];
+      }
     }
     __syncthreads();
     // operands: lane holds row (lane % 16), k = 8 (lane / 16) .. +7
-    bf16x8 a[2], b[2];
+    bf16x8 a[FM], b[FN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < FM; ++i)
       a[i] = *reinterpret_cast<const bf16x8*>(&As[(wm + i * 16 + (lane & 15)) * LDK + 8 * (lane >> 4)]);
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < FN; ++j)
       b[j] = *reinterpret_cast<const bf16x8*>(&Bs[(wn + j * 16 + (lane & 15)) * LDK + 8 * (lane >> 4)]);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < FN; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     __syncthreads();
   }
   // result: lane holds rows 4 (lane / 16) + r, column lane % 16 of each 16 x 16 block
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
@@ -223,6 +232,7 @@ using namespace mlp;
 
 struct memb_mlp {
   int device = 0;
+  int cus = 256;  // compute units of the device (GEMM tile choice)
   hipStream_t own = nullptr, stream = nullptr;
   std::string err;
   int din = 0, width = 0, nh = 0, dout = 0, maxb = 0;
@@ -272,9 +282,14 @@ template <bool TA, bool TB, int EPI>
 int gemm(memb_mlp* m, int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb,
          const float* bias, const float* aux, float* out0, void* out1, int ldc) {
   if (M % 64 || N % 64 || K % 32) return mfail(m, MEMB_E_ARG, "gemm shape not padded");
-  dim3 grid(N / TN, M / TM);
-  k_gemm<TA, TB, EPI><<<grid, THREADS, 0, m->stream>>>(M, N, K, A, lda, B, ldb, bias, aux, out0,
-                                                       out1, ldc);
+  if ((long long)(M / TM) * (N / TN) < 2LL * m->cus) {
+    // too few 64 x 64 tiles to fill the compute units (training batches): 32 x 32 tiles
+    k_gemm<32, 32, TA, TB, EPI><<<dim3(N / 32, M / 32), THREADS, 0, m->stream>>>(
+        M, N, K, A, lda, B, ldb, bias, aux, out0, out1, ldc);
+  } else {
+    k_gemm<TM, TN, TA, TB, EPI><<<dim3(N / TN, M / TM), THREADS, 0, m->stream>>>(
+        M, N, K, A, lda, B, ldb, bias, aux, out0, out1, ldc);
+  }
   XCHK(m, hipGetLastError());
   return MEMB_OK;
 }
@@ -349,6 +364,7 @@ int memb_mlp_create(memb_mlp** out, int din, int width, int nhidden, int dout, i
   XCHK(m, hipSetDevice(device));
   XCHK(m, hipStreamCreate(&m->own));
   m->stream = m->own;
+  (void)hipDeviceGetAttribute(&m->cus, hipDeviceAttributeMultiprocessorCount, device);
   m->din = din, m->width = width, m->nh = nhidden, m->dout = dout, m->maxb = max_batch;
   m->Dp = pad64(din), m->Wp = pad64(width), m->Op = pad64(dout), m->Mp = pad64(max_batch);
   m->nl = nhidden + 1;

@@ -1,4 +1,5 @@
-"""CPU: the synthetic workloads' geometry (capsule counts, points, volume fraction, spacing).
+"""CPU: the synthetic workloads' geometry (capsule counts, points, volume fraction, spacing),
+and the host-side checks of device tensors.
 
 The benchmark and the GPU tests regenerate these inputs from seeds on every machine, so the
 figures they are quoted on are pinned here.
@@ -49,3 +50,22 @@ def test_lattice_rejects_overfull():
     except ValueError:
         return
     raise AssertionError("a lattice denser than 2 r + gap was accepted")
+
+
+def test_device_tensor_check_rejects_before_any_library_call():
+    """The Python device wrappers check a tensor's placement, dtype, contiguity and size
+    before handing its raw pointer to the library (which cannot); on a CPU-only machine
+    every tensor is rejected for its placement."""
+    import pytest
+    import torch
+
+    from suguru_amd.interp_spread import _dev_check
+
+    with pytest.raises(ValueError, match="on the GPU"):
+        _dev_check("x", torch.zeros(8, dtype=torch.float64), 8)
+    with pytest.raises(ValueError, match="torch.float64"):
+        _dev_check("x", torch.zeros(8, dtype=torch.float32), 8)
+    with pytest.raises(ValueError, match="torch.int64"):
+        _dev_check("cap_off", torch.zeros(3, dtype=torch.int32), 3, integer=True)
+    with pytest.raises(ValueError):
+        _dev_check("x", [0.0] * 8, 8)

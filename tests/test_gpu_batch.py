@@ -331,3 +331,23 @@ def test_device_wrappers_reject_wrong_sizes(gpu_ctx_factory):
             b.bind(**bufs)
     finally:
         b.close()
+
+
+def test_set_config_of_the_one_call_entry_points(gpu_ctx_factory):
+    """ibx_set_config: an unknown configuration is rejected; every configuration, chosen by
+    name for the one-call entry points, gives the oracle's bits."""
+    _torch()
+    from suguru_amd import Batch, IbxError
+
+    wl = W.config_small()
+    g, caps = wl.grid, wl.caps
+    ctx = gpu_ctx_factory(g)
+    with pytest.raises(IbxError):
+        ctx.set_config(len(Batch.config_names()))
+    ref = O.lagr2eulr(g, caps.cap_off, caps.x, caps.y, caps.z, wl.fx, wl.fy, wl.fz, caps.area)
+    for name in Batch.config_names():
+        print("case: one-call spread, config", name, flush=True)
+        ctx.set_config(name)
+        got = ctx.lagr2eulr(caps.cap_off, caps.x, caps.y, caps.z, wl.fx, wl.fy, wl.fz, caps.area)
+        for a, b, nm in zip(got, ref, ("forcex", "forcey", "forcez")):
+            assert np.array_equal(bits(a), bits(b)), (name, nm)

@@ -89,3 +89,29 @@ def test_capsule_step_chain_matches_oracle():
             assert np.array_equal(bits(st.umean.cpu().numpy()), bits(um)), k
     finally:
         st.close()
+
+
+def test_capsule_step_rejects_bad_bindings():
+    """Uneven capsules (one membrane plan serves one degree), short fields and a null stream
+    are refused before anything reaches the libraries."""
+    import torch
+
+    from suguru_amd.capsule_step import CapsuleStep
+
+    g = W.make_grid(64, 64, 64, h=1.0 / 10.6)
+    r = 10.6 * g.dx
+    caps = W.build_capsules([(3.0, 3.0, 3.0)], [r], [16])
+    with pytest.raises(ValueError, match="points of degree"):
+        CapsuleStep(g, np.array([0, caps.npts - 1], dtype=np.int64), 16, 6250.0, 1875.0, 1e-4)
+    st = CapsuleStep(g, caps.cap_off, 16, 6250.0, 1875.0, 1e-4)
+    try:
+        e = lambda n: torch.zeros(n, dtype=torch.float64, device=st.dev)
+        nf, n = g.ncells_halo, caps.npts
+        with pytest.raises(ValueError, match="forcez"):
+            st.bind(e(nf), e(nf), e(nf), e(n), e(n), e(n), e(nf), e(nf), e(nf - 1))
+        with pytest.raises(ValueError, match="non-null"):
+            st.bind(e(nf), e(nf), e(nf), e(n), e(n), e(n), e(nf), e(nf), e(nf), stream=0)
+        with pytest.raises(RuntimeError):
+            st.step()  # nothing bound
+    finally:
+        st.close()

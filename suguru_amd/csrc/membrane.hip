@@ -375,6 +375,10 @@ __device__ void law(const Geom& o, const double* __restrict__ R, double gs, doub
 // 128 threads per capsule and four capsules per CU beat 256 threads and two by 11 %: the
 // per-node phase takes RING x np nodes at a time (3 x 68 = 204 at L = 16), which one round of
 // 256 threads covers no better than two rounds of 128.
+// The transform loops carry `#pragma unroll 4`.  Their trip counts are run-time values, so
+// without the hint each iteration waits for its own table loads.  Each sum keeps its order,
+// so the output is bitwise unchanged; 1000 capsules at L = 16 take 1.32 instead of 1.47 ms
+// (unrolled by 8: 1.35 ms).
 template <int MODE>
 __global__ void __launch_bounds__(THREADS, 2)
     k_mem(Tables T, long long ncap, double gs, double eb, const double* __restrict__ x,
@@ -397,6 +401,7 @@ __global__ void __launch_bounds__(THREADS, 2)
       const int c = t / NN, i = (t / N) % N, m = t % N;
       const double* f = xyz[c] + i * np2;
       double cs = 0.0, sn = 0.0;
+      #pragma unroll 4
       for (int j = 0; j < np2; ++j) {
         const double v = f[j];
         cs += v * T.cs2[j * N + m];
@@ -409,6 +414,7 @@ __global__ void __launch_bounds__(THREADS, 2)
       const int c = t / NN, m = (t / N) % N, n = t % N;
       double sa = 0.0, sb = 0.0;
       if (n >= m)
+        #pragma unroll 4
         for (int i = 0; i < N; ++i) {
           const double p = T.w2[i] * T.Pc[(i * N + m) * N + n];
           const int u = (c * N + i) * N + m;
@@ -444,6 +450,7 @@ __global__ void __launch_bounds__(THREADS, 2)
         const double* Pd = T.Pfd + (((size_t)(i0 + r) * 5 + k) * N + m) * N;
         const double h = m == 0 ? 0.5 : 1.0;
         double sa = 0.0, sb = 0.0;
+        #pragma unroll 4
         for (int n = m; n < N; ++n) {
           sa += Pd[n] * ca[(c * N + m) * N + n];
           sb += Pd[n] * cb[(c * N + m) * N + n];
@@ -501,6 +508,7 @@ __global__ void __launch_bounds__(THREADS, 2)
         for (int t = tid; t < nr * 6 * N; t += THREADS) {
           const int r = t / (6 * N), fi = (t / N) % 6, m = t % N;
           double cs = 0.0, sn = 0.0;
+          #pragma unroll 4
           for (int j = 0; j < np; ++j) {
             const double v = F[((size_t)r * np + j) * 6 + fi];
             cs += v * T.csf[j * N + m];
@@ -525,6 +533,7 @@ __global__ void __launch_bounds__(THREADS, 2)
           const int m = t / N, n = t % N;
           double sa = 0.0, sb = 0.0;
           if (n >= m)
+            #pragma unroll 4
             for (int i = 0; i < nt; ++i) {
               const double p = T.w[i] * T.Pfa[((size_t)i * N + m) * N + n];
               const double* s = scr + (((size_t)i * 6 + fi) * N + m) * 2;
@@ -538,6 +547,7 @@ __global__ void __launch_bounds__(THREADS, 2)
           const int i = t / N, m = t % N;
           const double h = m == 0 ? 0.5 : 1.0;
           double sa = 0.0, sb = 0.0;
+          #pragma unroll 4
           for (int n = m; n < N; ++n) {
             const double p = T.Pc[(i * N + m) * N + n];
             sa += p * fc[2 * (m * N + n)];
@@ -554,6 +564,7 @@ __global__ void __launch_bounds__(THREADS, 2)
           const double h = m == 0 ? 0.5 : 1.0;
           const double* P = (dth ? T.Pcd : T.Pc) + (i * N + m) * N;
           double sa = 0.0, sb = 0.0;
+          #pragma unroll 4
           for (int n = m; n < N; ++n) {
             sa += P[n] * ca[(c * N + m) * N + n];
             sb += P[n] * cb[(c * N + m) * N + n];
@@ -567,6 +578,7 @@ __global__ void __launch_bounds__(THREADS, 2)
       for (int node = tid; node < nc; node += THREADS) {
         const int i = node / np2, j = node % np2;
         double v = 0.0;
+        #pragma unroll 4
         for (int m = 0; m < N; ++m) {
           const double A = tmp[2 * (i * N + m)], B = tmp[2 * (i * N + m) + 1];
           const double cm = T.cs2[j * N + m], sm = T.sn2[j * N + m];

@@ -504,18 +504,23 @@ __global__ void __launch_bounds__(THREADS, 2)
       }
       __syncthreads();
       if (MODE == 0) {
-        // longitude DFT of the ring's six outputs, degrees <= L only
-        for (int t = tid; t < nr * 6 * N; t += THREADS) {
-          const int r = t / (6 * N), fi = (t / N) % 6, m = t % N;
-          double cs = 0.0, sn = 0.0;
-          #pragma unroll 4
+        // longitude DFT of the ring's six outputs, degrees <= L only.  A thread sums three
+        // outputs for one (ring, m), so that each pair of table loads feeds six FMAs and the
+        // 2 x 3 x 17 sums of a ring group fit one round of 128 threads.
+        for (int t = tid; t < nr * 2 * N; t += THREADS) {
+          const int r = t / (2 * N), f0 = 3 * ((t / N) % 2), m = t % N;
+          double cs0 = 0.0, sn0 = 0.0, cs1 = 0.0, sn1 = 0.0, cs2 = 0.0, sn2 = 0.0;
           for (int j = 0; j < np; ++j) {
-            const double v = F[((size_t)r * np + j) * 6 + fi];
-            cs += v * T.csf[j * N + m];
-            sn += v * T.snf[j * N + m];
+            const double* v = F + ((size_t)r * np + j) * 6 + f0;
+            const double c = T.csf[j * N + m], s = T.snf[j * N + m];
+            cs0 += v[0] * c, sn0 += v[0] * s;
+            cs1 += v[1] * c, sn1 += v[1] * s;
+            cs2 += v[2] * c, sn2 += v[2] * s;
           }
-          double* s = scr + (((size_t)(i0 + r) * 6 + fi) * N + m) * 2;
-          s[0] = cs, s[1] = sn;
+          double* s = scr + (((size_t)(i0 + r) * 6 + f0) * N + m) * 2;
+          s[0] = cs0, s[1] = sn0;
+          s[2 * N] = cs1, s[2 * N + 1] = sn1;
+          s[4 * N] = cs2, s[4 * N + 1] = sn2;
         }
         __syncthreads();
       }

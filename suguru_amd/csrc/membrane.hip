@@ -530,6 +530,11 @@ __global__ void __launch_bounds__(THREADS, 2)
     //    grid, field by field; the area from the coarse shape (sph3.f90:186-260)
     double* fc = U;              // [N][N][2] coefficients of the current field
     double* tmp = U + 2 * NN;    // [N][N][2] per (i, m) partial sums
+    // the force grid's longitude tables [np2][N], staged once per capsule: the node loop of
+    // each of the twelve fields then waits on LDS instead of on two table loads per term
+    double* c2 = U + 4 * NN;
+    double* s2 = c2 + np2 * N;
+    for (int t = tid; t < np2 * N; t += THREADS) c2[t] = T.cs2[t], s2[t] = T.sn2[t];
     double* xt = scr;            // [nc][3] x_theta on the force grid (scratch reused)
     double* xp = scr + (size_t)nc * 3;  // [nc][3] x_phi
     for (int fi = MODE == 3 ? 6 : 0; fi < 12; ++fi) {
@@ -586,7 +591,7 @@ __global__ void __launch_bounds__(THREADS, 2)
         #pragma unroll 4
         for (int m = 0; m < N; ++m) {
           const double A = tmp[2 * (i * N + m)], B = tmp[2 * (i * N + m) + 1];
-          const double cm = T.cs2[j * N + m], sm = T.sn2[j * N + m];
+          const double cm = c2[j * N + m], sm = s2[j * N + m];
           if (fi < 9)
             v += A * cm - B * sm;
           else
@@ -736,7 +741,7 @@ int upload(memb_plan* p, const std::vector<double>& h, const double** dst) {
 size_t lds_bytes(const Tables& T) {
   const int N = T.N, NN = N * N;
   size_t work = std::max<size_t>(3 * NN * 2, (size_t)RING * 3 * 5 * N * 2 + (size_t)RING * T.np * 6);
-  work = std::max<size_t>(work, 4 * NN);
+  work = std::max<size_t>(work, 8 * NN);  // phase 3: fc, tmp and the two [2N][N] tables
   return (6 * NN + work) * sizeof(double);
 }
 

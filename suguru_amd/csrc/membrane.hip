@@ -445,17 +445,22 @@ __global__ void __launch_bounds__(THREADS, 2)
     double* F = AB + RING * 3 * 5 * N * 2;   // [RING][np][6]
     for (int i0 = 0; i0 < (MODE == 3 ? 0 : nt); i0 += RING) {
       const int nr = (nt - i0) < RING ? (nt - i0) : RING;
-      for (int t = tid; t < nr * 3 * 5 * N; t += THREADS) {
-        const int r = t / (15 * N), c = (t / (5 * N)) % 3, k = (t / N) % 5, m = t % N;
+      // a thread sums the three coordinates of one (ring, k, m): each table load feeds six FMAs
+      for (int t = tid; t < nr * 5 * N; t += THREADS) {
+        const int r = t / (5 * N), k = (t / N) % 5, m = t % N;
         const double* Pd = T.Pfd + (((size_t)(i0 + r) * 5 + k) * N + m) * N;
         const double h = m == 0 ? 0.5 : 1.0;
-        double sa = 0.0, sb = 0.0;
-        #pragma unroll 4
+        double sa0 = 0.0, sb0 = 0.0, sa1 = 0.0, sb1 = 0.0, sa2 = 0.0, sb2 = 0.0;
         for (int n = m; n < N; ++n) {
-          sa += Pd[n] * ca[(c * N + m) * N + n];
-          sb += Pd[n] * cb[(c * N + m) * N + n];
+          const double p = Pd[n];
+          sa0 += p * ca[m * N + n], sb0 += p * cb[m * N + n];
+          sa1 += p * ca[(N + m) * N + n], sb1 += p * cb[(N + m) * N + n];
+          sa2 += p * ca[(2 * N + m) * N + n], sb2 += p * cb[(2 * N + m) * N + n];
         }
-        AB[2 * t] = h * sa, AB[2 * t + 1] = h * sb;
+        const int t0 = ((r * 3) * 5 + k) * N + m, cs = 5 * N;  // AB slot of coordinate 0; stride
+        AB[2 * t0] = h * sa0, AB[2 * t0 + 1] = h * sb0;
+        AB[2 * (t0 + cs)] = h * sa1, AB[2 * (t0 + cs) + 1] = h * sb1;
+        AB[2 * (t0 + 2 * cs)] = h * sa2, AB[2 * (t0 + 2 * cs) + 1] = h * sb2;
       }
       __syncthreads();
       for (int t = tid; t < nr * np; t += THREADS) {

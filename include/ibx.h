@@ -100,6 +100,12 @@ const char* ibx_version(void);
 
 /* Roma-Peskin-Berger 3-point delta multiplied by h: kernel.f90:7-25.  Host FP64. */
 double ibx_kernel(double r);
+/* The same kernel.f90:7-25 on the device: out[q] = kernel(r[q]), q < n (device arrays), with
+ * the device function every interpolation and spread weight uses.  Asynchronous. */
+int ibx_kernel_dev(ibx_ctx* ctx, int64_t n, const double* r, double* out);
+/* kerneltest, kernel.f90:43-63, evaluated on the device with that device kernel, summed in
+ * the reference's order (k outer, i inner); waits and writes the sum to *sum (host). */
+int ibx_kerneltest_dev(ibx_ctx* ctx, double* sum);
 
 /* ---- host-array entry points (the Fortran drop-in calls these) -------------------------- */
 
@@ -126,7 +132,11 @@ int ibx_lagr2eulrind(ibx_ctx* ctx, int64_t ncap, const int64_t* cap_off, const d
 /* ---- HBM-resident entry points ---------------------------------------------------------- */
 /* Same semantics as above; every pointer (cap_off included) is device memory; work is
  * enqueued on the context stream and returns without waiting.  Domain errors found on the
- * device are reported by the next ibx_sync(). */
+ * device are reported by the next ibx_sync().  The spread and Update_Pos entry points check
+ * the device-held cap_off on the device first (cap_off[0] = 0, non-decreasing,
+ * cap_off[ncap] = npts); a table that fails makes every kernel that indexes through it
+ * return at once, and the next ibx_sync() reports IBX_E_ARG (the outputs are then
+ * undefined). */
 
 int ibx_interp_f64_dev(ibx_ctx* ctx, const double* u, const double* v, const double* w,
                        int64_t ncap, const int64_t* cap_off, int64_t npts, const double* x,
@@ -188,7 +198,10 @@ int ibx_batch_set_overlap(ibx_batch* b, int on);
 int ibx_batch_num_configs(void);
 const char* ibx_batch_config_name(int choice);
 /* Capture interp + spread on the context stream into a graph (use_graph = 1) or run eagerly
- * (use_graph = 0) on every ibx_batch_run. */
+ * (use_graph = 0) on every ibx_batch_run.  Several batches may live on one context: when a
+ * later capture or call grows (frees and reallocates) a context work buffer, the graphs
+ * captured before it are recaptured by their next ibx_batch_run, whose eager sizing pass
+ * is then that run's work. */
 int ibx_batch_capture(ibx_batch* b, int use_graph);
 /* Timed runs: `nslots` sets of HIP events on the context stream (start, after the
  * interpolation, before and after the tile spread kernel, end); ibx_batch_run then enqueues

@@ -99,6 +99,8 @@ def load() -> ctypes.CDLL:
         "ibx_sync": (ctypes.c_int, [P]),
         "ibx_version": (ctypes.c_char_p, []),
         "ibx_kernel": (D, [D]),
+        "ibx_kernel_dev": (ctypes.c_int, [P, I64, P, P]),
+        "ibx_kerneltest_dev": (ctypes.c_int, [P, ctypes.POINTER(D)]),
         "ibx_eulr2lagr": (ctypes.c_int, [P, P, P, P, I64, P, P, P, P, P, P, P]),
         "ibx_lagr2eulr": (ctypes.c_int, [P, I64, P, P, P, P, P, P, P, P, P, P, P]),
         "ibx_lagr2eulrind": (ctypes.c_int, [P, I64, P, P, P, P, P, P, P, P, P, P, P]),

@@ -89,10 +89,13 @@ struct ibx_ctx {
   // spread by capsule objects
   DevBuf oboxes, oitems, osrc, onsrc, ocover, ometa, onbm, ocnt, ooff, obinm;
   // staging for the host-array entry points
-  DevBuf s_f1, s_f2, s_f3, s_x, s_y, s_z, s_a1, s_a2, s_a3, s_a4, s_o1, s_o2, s_o3, s_off;
+  DevBuf s_f1, s_f2, s_f3, s_x, s_y, s_z, s_a1, s_a2, s_a3, s_a4, s_o1, s_o2, s_o3, s_off, s_kt;
   int cfg = kDefaultCfg;
   unsigned long long ecap = 0;  // entries capacity (elements)
   std::vector<ibx_batch*> batches;  // live batches, destroyed with the context
+  // bumped whenever a work buffer is freed and reallocated: a batch graph captured under an
+  // older generation holds freed pointers and is recaptured before its next replay
+  unsigned long long gen = 0;
 };
 
 struct ibx_batch {
@@ -107,6 +110,7 @@ struct ibx_batch {
   bool use_graph = false;
   bool captured = false;
   bool overlap = false;  // interpolation on the side stream, concurrent with the spread
+  unsigned long long gen = 0;  // context buffer generation the graph was captured under
   // timed slots: IBX_TIMED_EV events per slot; timed run n records into slot n % nslots
   std::vector<hipEvent_t> tev;
   long long runs = 0;
@@ -135,6 +139,7 @@ int ensure(ibx_ctx* c, DevBuf& b, size_t bytes) {
     HIPCHK(c, hipFree(b.p));
     b.p = nullptr;
     b.bytes = 0;
+    ++c->gen;  // captured graphs that reference the old buffer are now stale
   }
   size_t nb = std::max<size_t>(bytes, 256);
   HIPCHK(c, hipMalloc(&b.p, nb));
@@ -195,56 +200,107 @@ int enqueue_scan(ibx_ctx* c, const unsigned* counts, long long n, unsigned* off)
   return IBX_OK;
 }
 
+// Work buffers of the spread by capsule objects for one call (device pointers).
+struct ObjBufs {
+  int* boxes;                // 9 objects per capsule, 6 ints each
+  unsigned long long* nbm;   // capsule neighbour masks, ncap * nwd words
+  int* cnt;                  // items per capsule
+  unsigned long long* binm;  // coarse-bin capsule masks, nbins * nwd words
+  int* off;                  // item offsets per capsule
+  ObjItem* items;            // icap items
+  int* src;                  // icap * OB_NSRC source objects
+  int* nsrc;                 // icap source counts
+  unsigned* cover;           // coverage bitmap, 1 bit per cell
+  int* meta;                 // OB_META queue counters
+  long long icap;            // item capacity
+};
+
+// Sizes (bytes) of the object-spread work buffers for ncap capsules on the context's grid;
+// items are sized separately (ObjSizes::per_item bytes per item of capacity).
+struct ObjSizes {
+  size_t boxes, nbm, cnt, binm, off, cover, meta;
+  long long icap_want;  // the allocating path's item capacity
+  int nwd;
+  long long nw, nbw;
+  static constexpr size_t per_item = sizeof(ObjItem) + OB_NSRC * sizeof(int) + sizeof(int);
+  size_t fixed() const { return boxes + nbm + cnt + binm + off + cover + meta; }
+};
+
+ObjSizes obj_sizes(const ibx_ctx* c, long long ncap) {
+  ObjSizes z{};
+  const long long ncells = c->gd.sz * (long long)(c->grid.ktot + 2);
+  z.nw = (ncells + 31) / 32;
+  z.nwd = (int)((ncap + 63) / 64);
+  const long long nbins = (long long)((c->grid.itot + OB_BIN - 1) / OB_BIN) *
+                          ((c->grid.jtot + OB_BIN - 1) / OB_BIN) *
+                          ((c->grid.ktot + 2 + OB_BIN - 1) / OB_BIN);
+  z.nbw = nbins * z.nwd;
+  z.icap_want = std::max<long long>(1024, 64 * ncap + 1024);
+  z.boxes = (size_t)std::max<long long>(1, 9 * ncap) * 6 * sizeof(int);
+  z.nbm = (size_t)std::max<long long>(1, ncap * z.nwd) * 8;
+  z.cnt = (size_t)std::max<long long>(1, ncap) * sizeof(int);
+  z.binm = (size_t)std::max<long long>(1, z.nbw) * 8;
+  z.off = (size_t)std::max<long long>(1, ncap) * sizeof(int);
+  z.cover = (size_t)z.nw * sizeof(unsigned);
+  z.meta = OB_META * sizeof(int);
+  return z;
+}
+
+// The allocating path: context-owned buffers, grown on demand.
+int obj_bufs_alloc(ibx_ctx* c, long long ncap, ObjBufs& ob) {
+  const ObjSizes z = obj_sizes(c, ncap);
+  const long long icap = z.icap_want;
+  int rc;
+  if ((rc = ensure(c, c->oboxes, z.boxes)) || (rc = ensure(c, c->onbm, z.nbm)) ||
+      (rc = ensure(c, c->ocnt, z.cnt)) || (rc = ensure(c, c->obinm, z.binm)) ||
+      (rc = ensure(c, c->ooff, z.off)) ||
+      (rc = ensure(c, c->oitems, (size_t)icap * sizeof(ObjItem))) ||
+      (rc = ensure(c, c->osrc, (size_t)icap * OB_NSRC * sizeof(int))) ||
+      (rc = ensure(c, c->onsrc, (size_t)icap * sizeof(int))) ||
+      (rc = ensure(c, c->ocover, z.cover)) || (rc = ensure(c, c->ometa, z.meta)))
+    return rc;
+  ob.boxes = (int*)c->oboxes.p, ob.nbm = (unsigned long long*)c->onbm.p;
+  ob.cnt = (int*)c->ocnt.p, ob.binm = (unsigned long long*)c->obinm.p;
+  ob.off = (int*)c->ooff.p, ob.items = (ObjItem*)c->oitems.p;
+  ob.src = (int*)c->osrc.p, ob.nsrc = (int*)c->onsrc.p;
+  ob.cover = (unsigned*)c->ocover.p, ob.meta = (int*)c->ometa.p;
+  ob.icap = icap;
+  return IBX_OK;
+}
+
 // The spread by capsule objects on the context stream: bitmap and counters, object boxes,
 // items and bitmap, capsule neighbours, sources, the persistent item / zero-unit kernel,
 // y halos (x halos are written with the cells).  ev_tile / ev_tile_end (optional) bracket the persistent
-// kernel; ev_split is recorded with ev_tile_end.
+// kernel; ev_split is recorded with ev_tile_end.  Allocates nothing: ob holds the buffers.
 int enqueue_spread_obj(ibx_ctx* c, const TileCfg& cf, bool cc, long long ncap,
                        const long long* d_off, long long npts, const double* x,
                        const double* y, const double* z, const double* f1, const double* f2,
                        const double* f3, const double* area, double* o1, double* o2, double* o3,
-                       hipEvent_t ev_tile, hipEvent_t ev_tile_end, hipEvent_t ev_split,
-                       hipEvent_t ev_before) {
+                       const ObjBufs& ob, hipEvent_t ev_tile, hipEvent_t ev_tile_end,
+                       hipEvent_t ev_split, hipEvent_t ev_before) {
   if (npts >= OB_MAXPTS) return fail(c, IBX_E_ARG, "object spread: more than 2^27 points");
   const int klo = cc ? 0 : 1, khi = cc ? c->grid.ktot + 1 : c->grid.ktot;
   const long long ncells = c->gd.sz * (long long)(c->grid.ktot + 2);
-  const long long icap = std::max<long long>(1024, 64 * ncap + 1024);
-  const long long nw = (ncells + 31) / 32;
-  const long long nobj = std::max<long long>(1, 9 * ncap);
-  const int nwd = (int)((ncap + 63) / 64);
-  const long long nbins = (long long)((c->grid.itot + OB_BIN - 1) / OB_BIN) *
-                          ((c->grid.jtot + OB_BIN - 1) / OB_BIN) *
-                          ((c->grid.ktot + 2 + OB_BIN - 1) / OB_BIN);
-  const long long nbw = nbins * nwd;
-  int rc;
-  if ((rc = ensure(c, c->oboxes, (size_t)nobj * 6 * sizeof(int))) ||
-      (rc = ensure(c, c->onbm, (size_t)std::max<long long>(1, ncap * nwd) * 8)) ||
-      (rc = ensure(c, c->ocnt, (size_t)std::max<long long>(1, ncap) * sizeof(int))) ||
-      (rc = ensure(c, c->obinm, (size_t)std::max<long long>(1, nbw) * 8)) ||
-      (rc = ensure(c, c->ooff, (size_t)std::max<long long>(1, ncap) * sizeof(int))) ||
-      (rc = ensure(c, c->oitems, (size_t)icap * sizeof(ObjItem))) ||
-      (rc = ensure(c, c->osrc, (size_t)icap * OB_NSRC * sizeof(int))) ||
-      (rc = ensure(c, c->onsrc, (size_t)icap * sizeof(int))) ||
-      (rc = ensure(c, c->ocover, (size_t)nw * sizeof(unsigned))) ||
-      (rc = ensure(c, c->ometa, OB_META * sizeof(int))))
-    return rc;
-  int* meta = (int*)c->ometa.p;
-  unsigned* cover = (unsigned*)c->ocover.p;
+  const ObjSizes zs = obj_sizes(c, ncap);
+  const long long nw = zs.nw, nbw = zs.nbw, icap = ob.icap;
+  const int nwd = zs.nwd;
+  int* meta = ob.meta;
+  unsigned* cover = ob.cover;
   k_obj_clear<<<blocks_for(std::max<long long>(std::max(nw, nbw), OB_META), 256), 256, 0,
-                c->stream>>>(nw, cover, nbw, (unsigned long long*)c->obinm.p, meta);
+                c->stream>>>(nw, cover, nbw, ob.binm, meta);
   if (npts > 0 && ncap > 0) {
     k_obj_boxes<<<(unsigned)ncap, 256, 0, c->stream>>>(
-        c->gd, klo, khi, d_off, x, y, z, (int*)c->oboxes.p, (int*)c->ocnt.p,
-        nwd, (unsigned long long*)c->obinm.p, c->d_err);
-    k_obj_scan<<<1, 1024, 0, c->stream>>>(ncap, (const int*)c->ocnt.p, (int*)c->ooff.p,
+        c->gd, klo, khi, d_off, x, y, z, ob.boxes, ob.cnt,
+        nwd, ob.binm, c->d_err);
+    k_obj_scan<<<1, 1024, 0, c->stream>>>(ncap, ob.cnt, ob.off,
                                           (int)icap, meta, c->d_err);
     k_obj_items<<<(unsigned)ncap, 256, 0, c->stream>>>(
-        c->gd, ncap, nwd, (const int*)c->oboxes.p, (const unsigned long long*)c->obinm.p,
-        (const int*)c->ooff.p, (ObjItem*)c->oitems.p, (int)icap, cover,
-        (unsigned long long*)c->onbm.p);
+        c->gd, ncap, nwd, ob.boxes, ob.binm,
+        ob.off, ob.items, (int)icap, cover,
+        ob.nbm, c->d_err);
     k_obj_sources<<<(unsigned)(c->num_cus * 4), 256, 0, c->stream>>>(
-        nwd, (const int*)c->oboxes.p, (const unsigned long long*)c->onbm.p,
-        (const ObjItem*)c->oitems.p, (int)icap, (int*)c->osrc.p, (int*)c->onsrc.p, meta,
+        nwd, ob.boxes, ob.nbm,
+        ob.items, (int)icap, ob.src, ob.nsrc, meta,
         c->d_err);
   }
   // another stream's work (the interpolation) joins here, before the persistent kernel
@@ -254,8 +310,8 @@ int enqueue_spread_obj(ibx_ctx* c, const TileCfg& cf, bool cc, long long ncap,
   {
     const unsigned nblk = (unsigned)((long long)c->num_cus * cf.wg_per_cu);
 #define OBJ_ARGS                                                                             \
-  c->gd, klo, khi, d_off, x, y, z, f1, f2, f3, area, (const ObjItem*)c->oitems.p, (int)icap, \
-      (const int*)c->osrc.p, (const int*)c->onsrc.p, (const unsigned*)cover, ncells, meta, o1, \
+  c->gd, klo, khi, d_off, x, y, z, f1, f2, f3, area, ob.items, (int)icap, \
+      ob.src, ob.nsrc, (const unsigned*)cover, ncells, meta, o1, \
       o2, o3, c->d_err
     if (cf.minw == 4) {
       if (cc)
@@ -290,9 +346,13 @@ int enqueue_spread(ibx_ctx* c, int cfgid, bool cc, long long ncap, const long lo
                    hipEvent_t ev_tile = nullptr, hipEvent_t ev_split = nullptr,
                    hipEvent_t ev_tile_end = nullptr, hipEvent_t ev_before = nullptr) {
   const TileCfg& cf = kCfgs[cfgid];
-  if (cf.obj)
+  if (cf.obj) {
+    ObjBufs ob;
+    int rc = obj_bufs_alloc(c, ncap, ob);
+    if (rc) return rc;
     return enqueue_spread_obj(c, cf, cc, ncap, d_off, npts, x, y, z, f1, f2, f3, area, o1, o2,
-                              o3, ev_tile, ev_tile_end, ev_split, ev_before);
+                              o3, ob, ev_tile, ev_tile_end, ev_split, ev_before);
+  }
   TileGeom tg = geom(c->grid, cf, cc);
   int rc;
   if ((rc = ensure(c, c->tcount, (size_t)tg.ntiles * sizeof(unsigned)))) return rc;
@@ -383,6 +443,9 @@ int collect_errors(ibx_ctx* c) {
   if (h) {
     k_fill_i32<<<1, 64, 0, c->stream>>>(c->d_err, 1, 0);
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (h & ERR_OFFSETS)
+      return fail(c, IBX_E_ARG,
+                  "cap_off (device) is not 0 at the start, non-decreasing and npts at the end");
     if (h & ERR_NONFINITE) return fail(c, IBX_E_DOMAIN, "non-finite Lagrangian position");
     if (h & ERR_FORCE) return fail(c, IBX_E_DOMAIN, "non-finite Lagrangian force or area");
     if (h & ERR_CAPACITY)
@@ -424,6 +487,30 @@ double ibx_kernel(double r) {
   return (1.0 / 3.0) * (1.0 + sqrt(-(3.0 * (r * r)) + 1.0));
 }
 
+int ibx_kernel_dev(ibx_ctx* c, int64_t n, const double* r, double* out) {
+  if (!c) return IBX_E_ARG;
+  if (n < 0) return fail(c, IBX_E_ARG, "negative size");
+  if (n > 0 && !(r && out)) return fail(c, IBX_E_ARG, "null pointer");
+  if (n == 0) return IBX_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  k_phi<<<blocks_for(n, 256), 256, 0, c->stream>>>(n, r, out);
+  HIPCHK(c, hipGetLastError());
+  return IBX_OK;
+}
+
+int ibx_kerneltest_dev(ibx_ctx* c, double* sum) {
+  if (!c) return IBX_E_ARG;
+  if (!sum) return fail(c, IBX_E_ARG, "null pointer");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = ensure(c, c->s_kt, sizeof(double));
+  if (rc) return rc;
+  k_kerneltest<<<1, 64, 0, c->stream>>>((double*)c->s_kt.p);
+  HIPCHK(c, hipGetLastError());
+  if ((rc = download(c, sum, c->s_kt, sizeof(double)))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return IBX_OK;
+}
+
 int ibx_create(ibx_ctx** out, const ibx_grid* g, int device) {
   if (!out || !g) return IBX_E_ARG;
   *out = nullptr;
@@ -456,6 +543,11 @@ int ibx_create(ibx_ctx** out, const ibx_grid* g, int device) {
       c->num_cus = prop.multiProcessorCount;
   }
   if (e != hipSuccess) {
+    // release what was created, in the order of ibx_destroy
+    if (c->d_err) (void)hipFree(c->d_err);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
     return IBX_E_HIP;
@@ -473,7 +565,8 @@ void ibx_destroy(ibx_ctx* c) {
   DevBuf* all[] = {&c->oboxes, &c->oitems, &c->osrc, &c->onsrc, &c->ocover, &c->ometa, &c->onbm, &c->ocnt, &c->ooff, &c->obinm,
                    &c->tcount, &c->toff, &c->tpart, &c->tlist, &c->tmeta, &c->entries, &c->s_f1, &c->s_f2,
                    &c->s_f3,   &c->s_x,  &c->s_y,   &c->s_z,     &c->s_a1, &c->s_a2,
-                   &c->s_a3,   &c->s_a4, &c->s_o1,  &c->s_o2,    &c->s_o3, &c->s_off};
+                   &c->s_a3,   &c->s_a4, &c->s_o1,  &c->s_o2,    &c->s_o3, &c->s_off,
+                   &c->s_kt};
   for (DevBuf* b : all) release(*b);
   if (c->d_err) (void)hipFree(c->d_err);
   if (c->side) (void)hipStreamSynchronize(c->side);
@@ -526,18 +619,30 @@ static bool aligned16(const double* a, const double* b, const double* c) {
   return (((uintptr_t)a | (uintptr_t)b | (uintptr_t)c) & 15) == 0;
 }
 
-static int spread_dev(ibx_ctx* c, bool cc, int64_t ncap, const int64_t* cap_off, int64_t npts,
-                      const double* x, const double* y, const double* z, const double* f1,
-                      const double* f2, const double* f3, const double* area, double* o1,
-                      double* o2, double* o3) {
-  if (!c) return IBX_E_ARG;
+static int spread_dev_args(ibx_ctx* c, int64_t ncap, const int64_t* cap_off, int64_t npts,
+                           const double* x, const double* y, const double* z, const double* f1,
+                           const double* f2, const double* f3, const double* area,
+                           const double* o1, const double* o2, const double* o3) {
   if (ncap < 0 || npts < 0) return fail(c, IBX_E_ARG, "negative size");
   if (npts >= (1ll << 28)) return fail(c, IBX_E_ARG, "more than 2^28 - 1 points");
   if (!(o1 && o2 && o3)) return fail(c, IBX_E_ARG, "null output field");
   if (!aligned16(o1, o2, o3)) return fail(c, IBX_E_ARG, "output fields not 16-byte aligned");
   if (npts > 0 && !(cap_off && x && y && z && f1 && f2 && f3 && area && ncap > 0))
     return fail(c, IBX_E_ARG, "null pointer or no capsules");
+  return IBX_OK;
+}
+
+static int spread_dev(ibx_ctx* c, bool cc, int64_t ncap, const int64_t* cap_off, int64_t npts,
+                      const double* x, const double* y, const double* z, const double* f1,
+                      const double* f2, const double* f3, const double* area, double* o1,
+                      double* o2, double* o3) {
+  if (!c) return IBX_E_ARG;
+  int rc = spread_dev_args(c, ncap, cap_off, npts, x, y, z, f1, f2, f3, area, o1, o2, o3);
+  if (rc) return rc;
   HIPCHK(c, hipSetDevice(c->device));
+  // the offset table lives in HBM: checked on the device before anything indexes through it
+  if (ncap > 0)
+    k_check_off<<<1, 256, 0, c->stream>>>(ncap, (const long long*)cap_off, npts, c->d_err);
   return enqueue_spread(c, c->cfg, cc, ncap, (const long long*)cap_off, npts, x, y, z, f1, f2,
                         f3, area, o1, o2, o3, true);
 }
@@ -659,11 +764,12 @@ int ibx_update_pos_dev(ibx_ctx* c, int64_t ncap, const int64_t* cap_off, int64_t
   if ((xold == nullptr) != (yold == nullptr))
     return fail(c, IBX_E_ARG, "xold and yold must both be given or both be NULL");
   HIPCHK(c, hipSetDevice(c->device));
+  k_check_off<<<1, 256, 0, c->stream>>>(ncap, (const long long*)cap_off, npts, c->d_err);
   if (npts > 0)
     k_advance<<<blocks_for(npts, 256), 256, 0, c->stream>>>(c->gd, c->grid.lz, dt, npts, x, y,
                                                             z, xold, yold, ul, vl, wl);
   k_centroid<<<(unsigned)ncap, 256, 0, c->stream>>>(c->gd, ncap, (const long long*)cap_off, x,
-                                                    y, z, ul, vl, wl, centroid, umean);
+                                                    y, z, ul, vl, wl, centroid, umean, c->d_err);
   HIPCHK(c, hipGetLastError());
   return IBX_OK;
 }
@@ -881,6 +987,7 @@ static int capture_common(ibx_batch* b, int use_graph, int nslots) {
   }
   b->runs = 0;
   b->captured = true;
+  b->gen = c->gen;
   return IBX_OK;
 }
 
@@ -904,6 +1011,14 @@ int ibx_batch_run(ibx_batch* b) {
   }
   ++b->runs;
   if (b->use_graph) {
+    if (b->gen != c->gen) {
+      // another batch or call on this context reallocated a work buffer this graph points
+      // to: capture again (its eager sizing pass is this run's work)
+      const long long runs = b->runs;
+      int rc = capture_common(b, 1, 0);
+      b->runs = runs;
+      return rc;
+    }
     HIPCHK(c, hipGraphLaunch(b->exec, c->stream));
     return IBX_OK;
   }

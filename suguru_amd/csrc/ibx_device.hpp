@@ -32,7 +32,23 @@ struct TileGeom {
   long long ntiles;
 };
 
-enum : int { ERR_NONFINITE = 1, ERR_CAPACITY = 2, ERR_FORCE = 4 };
+enum : int { ERR_NONFINITE = 1, ERR_CAPACITY = 2, ERR_FORCE = 4, ERR_OFFSETS = 8 };
+
+// A device-held capsule offset table that is not 0 at the start, non-decreasing and npts at
+// the end sets ERR_OFFSETS; every kernel that indexes points through the table (or through
+// work lists built from it) then returns at once, so no point index leaves [0, npts).
+__device__ __forceinline__ bool offsets_bad(const int* err) {
+  return (*(volatile const int*)err & ERR_OFFSETS) != 0;
+}
+
+__global__ void __launch_bounds__(256) k_check_off(long long ncap,
+                                                   const long long* __restrict__ off,
+                                                   long long npts, int* __restrict__ err) {
+  int bad = 0;
+  for (long long p = threadIdx.x; p < ncap; p += 256) bad |= off[p + 1] < off[p];
+  if (threadIdx.x == 0) bad |= off[0] != 0 || off[ncap] != npts;
+  if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(err, ERR_OFFSETS);
+}
 
 
 // kernel.f90:7-25, in the reference's operation order.  Both branches are evaluated and one
@@ -46,6 +62,30 @@ __device__ __forceinline__ double phi(double r) {
   const double sq = sqrt(arg);
   const double v = outer ? (1.0 / 6.0) * ((5.0 - 3.0 * ar) - sq) : (1.0 / 3.0) * (1.0 + sq);
   return ar > 1.5 ? 0.0 : v;
+}
+
+// kernel.f90:7-25 over an array: the phi() of every interpolation and spread weight.
+__global__ void __launch_bounds__(256) k_phi(long long n, const double* __restrict__ r,
+                                             double* __restrict__ out) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) out[q] = phi(r[q]);
+}
+
+// kerneltest, kernel.f90:43-63: the 51 kernel values phi(i/12), i = -25..25, are formed by
+// the lanes of one wavefront; lane 0 then adds the 51^3 products k -> j -> i in the
+// reference's order, each product formed left to right, (1/12)(1/12)(1/12) phi_x phi_y phi_z.
+__global__ void __launch_bounds__(64) k_kerneltest(double* __restrict__ sum) {
+  __shared__ double ph[51];
+  const int t = threadIdx.x;
+  if (t < 51) ph[t] = phi((double)(t - 25) / 12.0);
+  __syncthreads();
+  if (t != 0) return;
+  const double c = 1.0 / 12.0;
+  double s = 0.0;
+  for (int k = 0; k < 51; ++k)
+    for (int j = 0; j < 51; ++j)
+      for (int i = 0; i < 51; ++i) s = s + c * c * c * ph[i] * ph[j] * ph[k];
+  *sum = s;
 }
 
 // Image slots of a master that is its own neighbour (interp_spread.f90:82-141):
@@ -290,6 +330,7 @@ __global__ void __launch_bounds__(256) k_bin(GridD g, TileGeom tg, long long nca
                                              unsigned long long* __restrict__ entries,
                                              unsigned long long ecap,
                                              int* __restrict__ err) {
+  if (offsets_bad(err)) return;
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   bool live = q < npts;
   double xq = 0, yq = 0, zq = 0;
@@ -950,6 +991,7 @@ __global__ void __launch_bounds__(256) k_obj_boxes(GridD g, int klo, int khi,
                                                    int* __restrict__ cnt, int nwd,
                                                    unsigned long long* __restrict__ binm,
                                                    int* __restrict__ err) {
+  if (offsets_bad(err)) return;
   __shared__ int red[4][9][6];
   __shared__ int ncnt[9];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1040,6 +1082,7 @@ __global__ void __launch_bounds__(1024) k_obj_scan(long long ncap, const int* __
                                                    int* __restrict__ off, int icap,
                                                    int* __restrict__ meta,
                                                    int* __restrict__ err) {
+  if (offsets_bad(err)) return;
   __shared__ int ws[16];
   __shared__ int carry;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -1097,7 +1140,9 @@ __global__ void __launch_bounds__(256) k_obj_items(GridD g, long long ncap, int 
                                                    const int* __restrict__ off,
                                                    ObjItem* __restrict__ items, int icap,
                                                    unsigned* __restrict__ cover,
-                                                   unsigned long long* __restrict__ nbm) {
+                                                   unsigned long long* __restrict__ nbm,
+                                                   const int* __restrict__ err) {
+  if (offsets_bad(err)) return;
   __shared__ int B[9][6];
   const int tid = threadIdx.x;
   const long long p = blockIdx.x;
@@ -1214,6 +1259,7 @@ __global__ void __launch_bounds__(256) k_obj_sources(int nwd, const int* __restr
                                                      int* __restrict__ nsrc,
                                                      int* __restrict__ meta,
                                                      int* __restrict__ err) {
+  if (offsets_bad(err)) return;
   const int lane = threadIdx.x & 63;
   const int nitems = min(meta[0], icap);
   const int waves = gridDim.x * (blockDim.x >> 6);
@@ -1610,6 +1656,7 @@ __global__ void __launch_bounds__(OB_THREADS, MINW)
                  const unsigned* __restrict__ cover, long long ncells,
                  int* __restrict__ meta, double* __restrict__ o1, double* __restrict__ o2,
                  double* __restrict__ o3, int* __restrict__ err) {
+  if (offsets_bad(err)) return;
   using O = ObjSpread<CC>;
   __shared__ typename O::Smem sm;
   const int tid = threadIdx.x;
@@ -1779,7 +1826,9 @@ __global__ void __launch_bounds__(256) k_centroid(GridD g, long long ncap,
                                                   const double* __restrict__ vl,
                                                   const double* __restrict__ wl,
                                                   double* __restrict__ cen,
-                                                  double* __restrict__ um) {
+                                                  double* __restrict__ um,
+                                                  const int* __restrict__ err) {
+  if (offsets_bad(err)) return;
   __shared__ double st[6][256];
   __shared__ double sum[6];
   const long long p = blockIdx.x;

@@ -208,7 +208,9 @@ class Surrogate:
         _, area = self.geometry(x, y, z)
         P = t.stack([x.view(ncap, -1), y.view(ncap, -1), z.view(ncap, -1)]).permute(1, 0, 2)
         ind = self.indicator(pred, P, area)
-        bad = ind > tol
+        # fail closed: a NaN indicator (NaN / Inf prediction, or an all-zero one: 0 / 0) and
+        # any non-finite predicted value send the capsule to the law
+        bad = ~(ind <= tol) | ~t.isfinite(pred).all(dim=1)
         f = pred.double().view(ncap, 3, self.npts).permute(1, 0, 2).reshape(3, -1).contiguous()
         nb = int(bad.sum().item())
         if 2 * nb >= ncap:

@@ -351,3 +351,125 @@ def test_set_config_of_the_one_call_entry_points(gpu_ctx_factory):
         got = ctx.lagr2eulr(caps.cap_off, caps.x, caps.y, caps.z, wl.fx, wl.fy, wl.fz, caps.area)
         for a, b, nm in zip(got, ref, ("forcex", "forcey", "forcez")):
             assert np.array_equal(bits(a), bits(b)), (name, nm)
+
+
+def test_two_batches_one_context_replay_after_growth(gpu_ctx_factory, moderate):
+    """Batches share the context's work buffers.  A second, larger batch grows them (frees and
+    reallocates); the first batch's graph, captured before, holds the freed addresses, so its
+    next run captures again: every replay still gives the oracle's bits."""
+    torch = _torch()
+    from suguru_amd import Batch
+
+    wl, ref_i, ref_s = moderate
+    g, caps = wl.grid, wl.caps
+    nsmall = 4
+    off_s = caps.cap_off[:nsmall + 1].copy()
+    ns = int(off_s[-1])
+    sub = lambda a: np.ascontiguousarray(np.asarray(a)[:ns])
+    ri_s = O.eulr2lagr(g, wl.u, wl.v, wl.w, off_s, sub(caps.x), sub(caps.y), sub(caps.z))
+    rs_s = O.lagr2eulr(g, off_s, sub(caps.x), sub(caps.y), sub(caps.z), sub(wl.fx), sub(wl.fy),
+                       sub(wl.fz), sub(caps.area))
+    ctx = gpu_ctx_factory(g)
+
+    def bufs(npts, src_caps, src_wl):
+        t = {k: to_dev(torch, getattr(wl, k)) for k in ("u", "v", "w")}
+        for k in ("fx", "fy", "fz"):
+            t[k] = to_dev(torch, np.asarray(getattr(src_wl, k))[:npts])
+        for k in ("x", "y", "z", "area"):
+            t[k] = to_dev(torch, np.asarray(getattr(src_caps, k))[:npts])
+        for k in ("ul", "vl", "wl"):
+            t[k] = torch.empty(npts, dtype=torch.float64, device="cuda:0")
+        for k in ("forcex", "forcey", "forcez"):
+            t[k] = torch.empty(g.ncells_halo, dtype=torch.float64, device="cuda:0")
+        return t
+
+    ta, tb = bufs(ns, caps, wl), bufs(caps.npts, caps, wl)
+    torch.cuda.synchronize()
+    a, b = Batch(ctx, off_s), Batch(ctx, caps.cap_off)
+    try:
+        a.bind(**ta)
+        b.bind(**tb)
+
+        def check(t, ri, rs, what):
+            for k, r in zip(("ul", "vl", "wl"), ri):
+                assert np.array_equal(bits(t[k].cpu().numpy()), bits(r)), (what, k)
+            for k, r in zip(("forcex", "forcey", "forcez"), rs):
+                assert np.array_equal(bits(t[k].cpu().numpy()), bits(r.reshape(-1, order="F"))), (what, k)
+
+        def run(batch, t, what):
+            for k in ("ul", "vl", "wl", "forcex", "forcey", "forcez"):
+                t[k].fill_(np.nan)
+            torch.cuda.synchronize()
+            print("case: two batches,", what, flush=True)
+            batch.run()
+            ctx.sync()
+
+        a.set_config(0), b.set_config(0)
+        a.capture(use_graph=True)        # small: sizes the work buffers for 4 capsules
+        run(a, ta, "small first replay")
+        check(ta, ri_s, rs_s, "small first")
+        b.capture(use_graph=True)        # large: frees and reallocates them
+        for rep in range(2):             # the small graph captures again, then replays
+            run(a, ta, f"small after growth {rep}")
+            check(ta, ri_s, rs_s, f"small after growth {rep}")
+        run(b, tb, "large")
+        check(tb, ref_i, ref_s, "large")
+        # the same with the tile spread, whose candidate list also grows
+        a.set_config(4), b.set_config(4)
+        a.capture(use_graph=True)
+        b.capture(use_graph=True)
+        run(a, ta, "tile small after tile large")
+        check(ta, ri_s, rs_s, "tile small")
+        run(b, tb, "tile large")
+        check(tb, ref_i, ref_s, "tile large")
+    finally:
+        a.close()
+        b.close()
+
+
+def test_device_offsets_checked_on_device(gpu_ctx_factory):
+    """ibx_spread_f64_dev / ibx_update_pos_dev read cap_off from HBM: a table whose last entry
+    is not npts, or that decreases, is caught on the device before any kernel indexes
+    through it (IBX_E_ARG at the next sync); a valid call afterwards gives the oracle's bits."""
+    torch = _torch()
+    from suguru_amd import IbxError
+
+    wl = W.config_moderate()
+    g, caps = wl.grid, wl.caps
+    ctx = gpu_ctx_factory(g)
+    ctx.set_stream(torch.cuda.current_stream().cuda_stream)
+    d = {k: to_dev(torch, getattr(wl, k)) for k in ("fx", "fy", "fz")}
+    for k in ("x", "y", "z", "area"):
+        d[k] = to_dev(torch, getattr(caps, k))
+    F = [torch.empty(g.ncells_halo, dtype=torch.float64, device="cuda:0") for _ in range(3)]
+    n = caps.npts
+    long_end = caps.cap_off.copy()
+    long_end[-1] = n + 4096
+    decreasing = caps.cap_off.copy()
+    decreasing[5] = decreasing[7] + 1
+    nonzero = caps.cap_off.copy()
+    nonzero[0] = -3
+    try:
+        for nm, bad in (("last != npts", long_end), ("decreasing", decreasing), ("first != 0", nonzero)):
+            print("case: device offsets,", nm, flush=True)
+            off = torch.from_numpy(bad).to("cuda:0")
+            ctx.spread_dev(off, d["x"], d["y"], d["z"], d["fx"], d["fy"], d["fz"], d["area"], *F)
+            with pytest.raises(IbxError) as ei:
+                ctx.sync()
+            assert ei.value.code == -1 and "cap_off" in str(ei.value)
+            ul = torch.zeros(n, dtype=torch.float64, device="cuda:0")
+            cen = torch.empty(3 * caps.ncap, dtype=torch.float64, device="cuda:0")
+            um = torch.empty(3 * caps.ncap, dtype=torch.float64, device="cuda:0")
+            xs, ys, zs = (d[k].clone() for k in "xyz")
+            ctx.update_pos_dev(off, xs, ys, zs, ul, ul, ul, 1e-4, cen, um)
+            with pytest.raises(IbxError) as ei:
+                ctx.sync()
+            assert ei.value.code == -1
+        off = torch.from_numpy(caps.cap_off).to("cuda:0")
+        ctx.spread_dev(off, d["x"], d["y"], d["z"], d["fx"], d["fy"], d["fz"], d["area"], *F)
+        ctx.sync()
+        ref = O.lagr2eulr(g, caps.cap_off, caps.x, caps.y, caps.z, wl.fx, wl.fy, wl.fz, caps.area)
+        for a, r in zip(F, ref):
+            assert np.array_equal(bits(a.cpu().numpy()), bits(r.reshape(-1, order="F")))
+    finally:
+        ctx.set_stream(0)

@@ -277,3 +277,38 @@ def test_dense_overlapping_capsules_capacity_and_fallback(gpu_ctx_factory):
             assert_bitwise(t[k].cpu().numpy(), r.reshape(-1, order="F"), k)
     finally:
         b.close()
+
+
+def test_device_kernel_sweep_and_kerneltest(gpu_ctx_factory):
+    """The device kernel (the one every weight uses) against the reference's own compiled
+    kernel.f90 values (tests/golden/kernel_ref.npz) and the oracle on a dense sweep that holds
+    the branch points |r| = 0.5 and 1.5 exactly and their neighbouring doubles; and kerneltest
+    (kernel.f90:43-63) summed on the device equals out.log:1 bit for bit."""
+    import ctypes
+    import os
+
+    import torch
+    from suguru_amd import _lib
+
+    lib = _lib.load()
+    ctx = gpu_ctx_factory(W.config_small(with_fields=False).grid)
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "kernel_ref.npz"))
+    edges = np.array([0.5, 1.5, -0.5, -1.5, 0.0, -0.0, 1.0, -1.0])
+    near = np.concatenate([np.nextafter(edges, np.inf), np.nextafter(edges, -np.inf)])
+    sweep = np.concatenate([gold["r"], edges, near, np.linspace(-2.0, 2.0, 100001),
+                            np.random.default_rng(9).uniform(-1.6, 1.6, 50000)])
+    r = torch.from_numpy(sweep).to("cuda:0")
+    out = torch.empty_like(r)
+    print(f"case: device kernel sweep, {sweep.size} arguments", flush=True)
+    p = lambda t: ctypes.c_void_p(int(t.data_ptr()))
+    assert lib.ibx_kernel_dev(ctx.h, int(r.numel()), p(r), p(out)) == 0
+    ctx.sync()
+    got = out.cpu().numpy()
+    ng = gold["r"].size
+    assert np.array_equal(bits(got[:ng]), bits(gold["kernel"]))
+    want = np.array([O.kernel(float(x)) for x in sweep[ng:]])
+    assert np.array_equal(bits(got[ng:]), bits(want))
+    print("case: device kerneltest", flush=True)
+    s = ctypes.c_double()
+    assert lib.ibx_kerneltest_dev(ctx.h, ctypes.byref(s)) == 0
+    assert s.value == float("0.99999999999996703") == float(gold["kerneltest"][0])

@@ -132,3 +132,32 @@ def test_partial_fallback_selects_per_capsule(trained, q):
     fv, lv = f.view(3, n, -1), f_law.view(3, n, -1)
     assert torch.equal(fv[:, bad], lv[:, bad])
     assert torch.equal(fv[:, ~bad], fp[:, ~bad])
+
+
+def test_gate_fails_closed_on_nan_inf_and_zero_predictions(trained, monkeypatch):
+    """A prediction row that is NaN, holds an Inf, or is all zero (indicator 0/0 = NaN) must
+    fall back to the FP64 law at any tolerance, even tol = inf: the capsule then carries the
+    law's traction bit for bit."""
+    torch = _torch()
+    s, Xt, _, Pt, _, _, _ = trained
+    x, y, z = (Pt[k].reshape(-1).contiguous() for k in range(3))
+    n = Xt.shape[0]
+    real_predict = s.predict
+    broken = {3: "nan", 17: "inf", 40: "zero"}
+
+    def predict(X):
+        p = real_predict(X).clone()
+        p[3] = float("nan")
+        p[17, 5] = float("inf")
+        p[40] = 0.0
+        return p
+
+    monkeypatch.setattr(s, "predict", predict)
+    f_law, _, _ = s.law(x, y, z)
+    for tol in (1e-3, float("inf")):
+        f, _, bad = s.step(x, y, z, tol)
+        for i in broken:
+            assert bool(bad[i]), (tol, broken[i])
+            assert torch.equal(f.view(3, n, -1)[:, i], f_law.view(3, n, -1)[:, i]), (tol, broken[i])
+        if tol == float("inf"):
+            assert int(bad.sum()) == len(broken)

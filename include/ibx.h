@@ -148,6 +148,21 @@ int ibx_spread_f64_dev(ibx_ctx* ctx, int64_t ncap, const int64_t* cap_off, int64
                        const double* fy, const double* fz, const double* area,
                        double* forcex, double* forcey, double* forcez);
 
+/* The plain spread (ibx_spread_f64_dev) allocating nothing: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under hipGraph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Its work buffers (object boxes, items, sources,
+ * coverage bitmap, capsule masks, queue counters) live in one fixed 64 MiB device array per
+ * GPU, held until the process ends; a call whose buffers do not fit returns IBX_E_ARG before
+ * any device work, and one that needs more work items than fit reports IBX_E_CAPACITY at the
+ * next ibx_sync.  Every call on a GPU, from any context, uses that GPU's one array, which
+ * keeps the last call's contents (each call clears what must start at zero): calls that
+ * overlap on one GPU (two streams, two host threads) race.  Same bits as ibx_spread_f64_dev. */
+int ibx_spread_f64_dev_noalloc(ibx_ctx* ctx, int64_t ncap, const int64_t* cap_off,
+                               int64_t npts, const double* x, const double* y,
+                               const double* z, const double* fx, const double* fy,
+                               const double* fz, const double* area, double* forcex,
+                               double* forcey, double* forcez);
+
 int ibx_spread_cc_f64_dev(ibx_ctx* ctx, int64_t ncap, const int64_t* cap_off, int64_t npts,
                           const double* x, const double* y, const double* z,
                           const double* xn, const double* yn, const double* zn,

@@ -107,6 +107,7 @@ def load() -> ctypes.CDLL:
         "ibx_interp_f64_dev": (ctypes.c_int, [P, P, P, P, I64, P, I64, P, P, P, P, P, P]),
         "ibx_spread_f64_dev": (ctypes.c_int, [P, I64, P, I64, P, P, P, P, P, P, P, P, P, P]),
         "ibx_spread_cc_f64_dev": (ctypes.c_int, [P, I64, P, I64, P, P, P, P, P, P, P, P, P, P]),
+        "ibx_spread_f64_dev_noalloc": (ctypes.c_int, [P, I64, P, I64, P, P, P, P, P, P, P, P, P, P]),
         "ibx_update_pos": (ctypes.c_int, [P, I64, P, D, P, P, P, P, P, P, P, P, P, P, P]),
         "ibx_update_pos_dev": (ctypes.c_int, [P, I64, P, I64, D, P, P, P, P, P, P, P, P, P, P]),
         "ibx_batch_create": (ctypes.c_int, [P, ctypes.POINTER(P), I64, P]),

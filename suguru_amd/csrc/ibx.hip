@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -84,6 +85,7 @@ struct ibx_ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string err;
   int* d_err = nullptr;
+  void* scratch = nullptr;  // this device's no-allocation work array (device address)
   // spread binning
   DevBuf tcount, toff, tpart, tlist, tmeta, entries;
   // spread by capsule objects
@@ -200,6 +202,35 @@ int enqueue_scan(ibx_ctx* c, const unsigned* counts, long long n, unsigned* off)
   return IBX_OK;
 }
 
+// ---- the fixed work array of the no-allocation spread ------------------------------------
+// ibx_spread_f64_dev_noalloc carves the object spread's work buffers out of this array
+// instead of allocating them.  Each GPU holds its own copy for the life of the process.  In
+// host code `scratch` is only a stand-in for the symbol: its device address is taken once per
+// device (hipGetSymbolAddress, at the first ibx_create on that device, with the device
+// current) and only that address is ever passed to kernels.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+// the device address of `scratch` on `device` (which must be current)
+hipError_t scratch_address(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_addr[device] = p;
+  }
+  *out = g_scratch_addr[device];
+  return hipSuccess;
+}
+
 // Work buffers of the spread by capsule objects for one call (device pointers).
 struct ObjBufs {
   int* boxes;                // 9 objects per capsule, 6 ints each
@@ -265,6 +296,42 @@ int obj_bufs_alloc(ibx_ctx* c, long long ncap, ObjBufs& ob) {
   ob.src = (int*)c->osrc.p, ob.nsrc = (int*)c->onsrc.p;
   ob.cover = (unsigned*)c->ocover.p, ob.meta = (int*)c->ometa.p;
   ob.icap = icap;
+  return IBX_OK;
+}
+
+// The no-allocation path: the same buffers carved out of this device's fixed work array, each
+// aligned to 256 bytes.  The item capacity is what the array has left (at most the allocating
+// path's); a call that needs more items than that reports IBX_E_CAPACITY at the next ibx_sync.
+int obj_bufs_scratch(ibx_ctx* c, long long ncap, ObjBufs& ob) {
+  if (!c->scratch) return fail(c, IBX_E_STATE, "no work array address for this device");
+  const ObjSizes z = obj_sizes(c, ncap);
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t fixed = up(z.boxes) + up(z.nbm) + up(z.cnt) + up(z.binm) + up(z.off) +
+                       up(z.cover) + up(z.meta);
+  const size_t min_items = 1024;
+  if (fixed + 3 * 256 + min_items * ObjSizes::per_item > kScratchBytes)
+    return fail(c, IBX_E_ARG,
+                "no-allocation spread: " + std::to_string(ncap) + " capsules on this grid need " +
+                    std::to_string((fixed + 3 * 256 + min_items * ObjSizes::per_item) >> 20) +
+                    " MiB of work buffers, more than the fixed " +
+                    std::to_string(kScratchBytes >> 20) + " MiB array");
+  const long long fit = (long long)((kScratchBytes - fixed - 3 * 256) / ObjSizes::per_item);
+  const long long icap = std::min<long long>(z.icap_want, fit);
+  unsigned char* p = (unsigned char*)c->scratch;
+  auto take = [&](size_t b) {
+    unsigned char* q = p;
+    p += up(b);
+    return (void*)q;
+  };
+  ob.boxes = (int*)take(z.boxes), ob.nbm = (unsigned long long*)take(z.nbm);
+  ob.cnt = (int*)take(z.cnt), ob.binm = (unsigned long long*)take(z.binm);
+  ob.off = (int*)take(z.off), ob.cover = (unsigned*)take(z.cover), ob.meta = (int*)take(z.meta);
+  ob.items = (ObjItem*)take((size_t)icap * sizeof(ObjItem));
+  ob.src = (int*)take((size_t)icap * OB_NSRC * sizeof(int));
+  ob.nsrc = (int*)take((size_t)icap * sizeof(int));
+  ob.icap = icap;
+  if ((size_t)(p - (unsigned char*)c->scratch) > kScratchBytes)
+    return fail(c, IBX_E_ARG, "no-allocation spread: work buffers overrun the fixed array");
   return IBX_OK;
 }
 
@@ -534,6 +601,7 @@ int ibx_create(ibx_ctx** out, const ibx_grid* g, int device) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
+  if (e == hipSuccess) e = scratch_address(device, &c->scratch);
   if (e == hipSuccess) e = hipMalloc(&c->d_err, sizeof(int));
   if (e == hipSuccess) e = hipMemsetAsync(c->d_err, 0, sizeof(int), c->own);
   if (e == hipSuccess) e = hipStreamSynchronize(c->own);
@@ -653,6 +721,28 @@ int ibx_spread_f64_dev(ibx_ctx* c, int64_t ncap, const int64_t* cap_off, int64_t
                        double* forcey, double* forcez) {
   return spread_dev(c, false, ncap, cap_off, npts, x, y, z, fx, fy, fz, area, forcex, forcey,
                     forcez);
+}
+
+int ibx_spread_f64_dev_noalloc(ibx_ctx* c, int64_t ncap, const int64_t* cap_off, int64_t npts,
+                               const double* x, const double* y, const double* z,
+                               const double* fx, const double* fy, const double* fz,
+                               const double* area, double* forcex, double* forcey,
+                               double* forcez) {
+  if (!c) return IBX_E_ARG;
+  int rc = spread_dev_args(c, ncap, cap_off, npts, x, y, z, fx, fy, fz, area, forcex, forcey,
+                           forcez);
+  if (rc) return rc;
+  ObjBufs ob;
+  if ((rc = obj_bufs_scratch(c, ncap, ob))) return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (ncap > 0)
+    k_check_off<<<1, 256, 0, c->stream>>>(ncap, (const long long*)cap_off, npts, c->d_err);
+  // the context's configuration when it is an object one, else the default object one (the
+  // tile spread sizes its candidate list per call)
+  const int cfg = kCfgs[c->cfg].obj ? c->cfg : kDefaultCfg;
+  return enqueue_spread_obj(c, kCfgs[cfg], false, ncap, (const long long*)cap_off, npts, x, y,
+                            z, fx, fy, fz, area, forcex, forcey, forcez, ob, nullptr, nullptr,
+                            nullptr, nullptr);
 }
 
 int ibx_spread_cc_f64_dev(ibx_ctx* c, int64_t ncap, const int64_t* cap_off, int64_t npts,

@@ -198,7 +198,13 @@ class IBTransfer:
                                                 _ptr(x), _ptr(y), _ptr(z), _ptr(ul), _ptr(vl),
                                                 _ptr(wl)))
 
-    def spread_dev(self, cap_off, x, y, z, fx, fy, fz, area, forcex, forcey, forcez, cc=False):
+    def spread_dev(self, cap_off, x, y, z, fx, fy, fz, area, forcex, forcey, forcez, cc=False,
+                   noalloc=False):
+        """lagr2eulr (cc=False) or lagr2eulrind (cc=True) on device tensors.  noalloc=True:
+        the plain spread through ibx_spread_f64_dev_noalloc, whose work buffers live in a
+        fixed device array (no allocation in the call; calls on one GPU must not overlap)."""
+        if cc and noalloc:
+            raise ValueError("the no-allocation entry point is the plain spread (cc=False)")
         n, nf = int(x.numel()), self.grid.ncells_halo
         for nm, t in (("x", x), ("y", y), ("z", z), ("fx", fx), ("fy", fy), ("fz", fz),
                       ("area", area)):
@@ -206,7 +212,8 @@ class IBTransfer:
         for nm, t in (("forcex", forcex), ("forcey", forcey), ("forcez", forcez)):
             _dev_check(nm, t, nf)
         _dev_check("cap_off", cap_off, int(cap_off.numel()), integer=True)
-        fn = self.lib.ibx_spread_cc_f64_dev if cc else self.lib.ibx_spread_f64_dev
+        fn = (self.lib.ibx_spread_cc_f64_dev if cc else
+              self.lib.ibx_spread_f64_dev_noalloc if noalloc else self.lib.ibx_spread_f64_dev)
         self._check(fn(self.h, int(cap_off.numel()) - 1, _ptr(cap_off), int(x.numel()), _ptr(x),
                        _ptr(y), _ptr(z), _ptr(fx), _ptr(fy), _ptr(fz), _ptr(area),
                        _ptr(forcex), _ptr(forcey), _ptr(forcez)))

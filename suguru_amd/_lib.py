@@ -194,3 +194,52 @@ def load_memb() -> ctypes.CDLL:
         f.argtypes = args
     _memb = lib
     return lib
+
+
+# ---- libfluid.so: the flow update (include/fluid.h) -------------------------------------------
+FLUID_PATH = os.path.join(HERE, "libfluid.so")
+FLUID_HEADER = os.path.join(ROOT, "include", "fluid.h")
+_fluid = None
+
+
+def declared_fluid_symbols(header: str = FLUID_HEADER):
+    """Names of the functions include/fluid.h declares."""
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fld_[a-z0-9_]+)\s*\(", text)))
+
+
+def load_fluid() -> ctypes.CDLL:
+    global _fluid
+    if _fluid is not None:
+        return _fluid
+    if not os.path.exists(FLUID_PATH):
+        raise ImportError(
+            f"{FLUID_PATH} is missing: build it with `python -m suguru_amd.build` "
+            "(there is no fallback implementation)")
+    torch_runtime_first()
+    lib = ctypes.CDLL(FLUID_PATH)
+    P = ctypes.c_void_p
+    D = ctypes.c_double
+    I = ctypes.c_int
+    sig = {
+        "fld_create": (I, [ctypes.POINTER(P), ctypes.POINTER(IbxGrid), D, D, I]),
+        "fld_destroy": (None, [P]),
+        "fld_last_error": (ctypes.c_char_p, [P]),
+        "fld_set_stream": (I, [P, P]),
+        "fld_sync": (I, [P]),
+        "fld_fftuvw_dev": (I, [P] + [P] * 13),
+        "fld_solver1d_dev": (I, [P, P, I]),
+        "fld_bounduvw_dev": (I, [P, P, P, P]),
+        "fld_boundp_dev": (I, [P, P]),
+        "fld_fillps_dev": (I, [P, P, P, P, P]),
+        "fld_correc_dev": (I, [P, P, P, P, P]),
+        "fld_step_dev": (I, [P] + [P] * 14),
+        "fld_chkdiv_dev": (I, [P, P, P, P, ctypes.POINTER(D), ctypes.POINTER(D)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _fluid = lib
+    return lib

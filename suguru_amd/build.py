@@ -1,4 +1,4 @@
-"""Build libibx.so (gfx950) in-tree with hipcc.
+"""Build libibx.so, libmemb.so and libfluid.so (gfx950) in-tree with hipcc.
 
     python -m suguru_amd.build            # or __graft_entry__.build()
 
@@ -20,6 +20,9 @@ DEPS = SOURCES + [os.path.join(HERE, "csrc", "ibx_device.hpp"), os.path.join(ROO
 # the capsule-membrane library: FP64 membrane law and the BF16-MFMA surrogate
 MEMB_LIB = os.path.join(HERE, "libmemb.so")
 MEMB_SOURCES = [os.path.join(HERE, "csrc", "membrane.hip"), os.path.join(HERE, "csrc", "mlp.hip")]
+# the flow update: FFT momentum and pressure solves (no FMA contraction, like the transfer)
+FLUID_LIB = os.path.join(HERE, "libfluid.so")
+FLUID_SOURCES = [os.path.join(HERE, "csrc", "fluid.hip")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("IBX_OFFLOAD_ARCH", "gfx950")
@@ -57,6 +60,7 @@ def _compile(cmd, out, verbose):
 def build(verbose: bool = True) -> str:
     _compile(hipcc_cmd(LIB), LIB, verbose)
     _compile(hipcc_cmd(MEMB_LIB, MEMB_SOURCES, contract=True), MEMB_LIB, verbose)
+    _compile(hipcc_cmd(FLUID_LIB, FLUID_SOURCES), FLUID_LIB, verbose)
     return LIB
 
 

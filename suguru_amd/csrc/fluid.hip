@@ -1,0 +1,871 @@
+// fluid.hip -- libfluid: the FFT flow update of one rank on gfx950 (declarations and reference
+// citations in include/fluid.h).
+//
+// Layout.  The caller's fields are Fortran (0:i1,0:j1,0:k1) arrays, i fastest.  The solves work
+// in compact spectral arrays of itot x jtot x ktot doubles (i fastest), one per velocity
+// component; the pressure solve reuses the first.  A solve is five passes:
+//   1. x transform of every (j, k) line (k_rfft, lines contiguous),
+//   2. y transform of every (i, k) line (k_rfft, lines = 16 neighbouring columns, so that each
+//      row segment is one 128-byte load),
+//   3. Gaussian elimination in k per (i, j) mode (k_tri: one lane per mode, the forward sweep
+//      kept in LDS, the per-mode reciprocal pivots read from a table built once per context),
+//   4. y inverse, 5. x inverse (into the caller's field interior).
+// Transforms.  A real line of length N = 2M is transformed as M complex values
+// z_n = x_2n + i x_2n+1 (radix-2, in LDS, input bit-reversed on load), then split into the
+// half spectrum X_0 .. X_M.  Packed real layout of a line: [X_0, X_M, Re X_1, Im X_1, ...,
+// Re X_M-1, Im X_M-1]; position p carries wavenumber 0 (p = 0), M (p = 1) or p / 2.  The
+// second-difference eigenvalue of a position depends only on its wavenumber, which is what the
+// reference's xrt/yrt tables encode for VFFTPACK's layout (initsolver.f90:108-132).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/fluid.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int FFT_THREADS = 256;
+constexpr int TRI_MODES = 64;        // modes (lanes) per elimination workgroup
+constexpr int Y_LINES = 16;          // columns per y-transform workgroup
+constexpr size_t LDS_MAX = 160 * 1024;
+
+struct FGrid {
+  int itot, jtot, ktot;
+  long long sy, sz;  // Fortran field pitches: row itot+2, plane (itot+2)(jtot+2)
+  double dxi, dyi, dzi;
+};
+
+__device__ __forceinline__ long long fidx(const FGrid& g, int i, int j, int k) {
+  return (long long)i + g.sy * j + g.sz * k;
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// ---- transforms ---------------------------------------------------------------------------
+// One workgroup transforms `nl` lines of length N = 2M of one k plane; element e of line l
+// lives at base + l * ls + e * es (source and destination each have their own geometry), the
+// workgroup's base being plane * (block / groups_per_plane) + gstride * (block % groups).
+// tw[k] = exp(-2 pi i k / N), k < M.  x lines: nl consecutive rows; y lines: nl consecutive
+// columns.
+
+// radix-2 decimation in time over nl lines of M complex values (input in bit-reversed order)
+template <bool INV>
+__device__ void fft_stages(double2* s, int nl, int M, int N, const double2* __restrict__ tw) {
+  const int half_m = M >> 1;
+  const int nb = nl * half_m;
+  for (int h = 1, lg = 0; h < M; h <<= 1, ++lg) {
+    __syncthreads();
+    const int tstride = N / (2 * h);
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+      const int line = b / half_m, r = b - line * half_m;
+      const int pos = r & (h - 1), grp = r >> lg;
+      const int i0 = line * M + (grp << (lg + 1)) + pos, i1 = i0 + h;
+      double2 w = tw[pos * tstride];
+      if (INV) w.y = -w.y;
+      const double2 a = s[i0], t = cmul(w, s[i1]);
+      s[i0] = make_double2(a.x + t.x, a.y + t.y);
+      s[i1] = make_double2(a.x - t.x, a.y - t.y);
+    }
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ int brev(int n, int logm) {
+  return logm ? (int)(__brev((unsigned)n) >> (32 - logm)) : 0;
+}
+
+// Forward: real lines -> packed half spectra.  XMAJOR: element index fastest across threads
+// (x lines); otherwise line index fastest (y lines).  src and dst may be the same array (the
+// workgroup reads all of its lines before the first barrier and writes them after the last).
+template <bool XMAJOR>
+__global__ void __launch_bounds__(FFT_THREADS)
+    k_rfft_fwd(const double* src, double* dst, int N, int logm,
+               int nl, int groups_per_plane, long long plane_src, long long plane_dst,
+               long long gstride_src, long long gstride_dst, long long es_src, long long ls_src,
+               long long es_dst, long long ls_dst, long long off_src,
+               const double2* __restrict__ tw) {
+  extern __shared__ double2 sm[];
+  const int M = N >> 1;
+  const int kp = blockIdx.x / groups_per_plane, gp = blockIdx.x - kp * groups_per_plane;
+  const long long bs = off_src + kp * plane_src + gp * gstride_src;
+  const long long bd = kp * plane_dst + gp * gstride_dst;
+  double* smd = reinterpret_cast<double*>(sm);
+  for (int q = threadIdx.x; q < nl * N; q += blockDim.x) {
+    const int l = XMAJOR ? q / N : q % nl, e = XMAJOR ? q - (q / N) * N : q / nl;
+    smd[2 * (l * M + brev(e >> 1, logm)) + (e & 1)] = src[bs + l * ls_src + e * es_src];
+  }
+  fft_stages<false>(sm, nl, M, N, tw);
+  // X_k = E_k + W^k O_k, E_k = (Z_k + conj Z_M-k) / 2, O_k = (Z_k - conj Z_M-k) / 2i
+  for (int q = threadIdx.x; q < nl * M; q += blockDim.x) {
+    const int l = XMAJOR ? q / M : q % nl, k = XMAJOR ? q - (q / M) * M : q / nl;
+    const double2* z = sm + l * M;
+    double* out = dst + bd + l * ls_dst;
+    if (k == 0) {
+      const double2 z0 = z[0];
+      out[0] = z0.x + z0.y;
+      out[es_dst] = z0.x - z0.y;
+    } else {
+      const double2 a = z[k], b = z[M - k];
+      const double2 E = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+      const double2 O = make_double2(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));
+      const double2 X = cmul(tw[k], O);
+      out[(2 * k) * es_dst] = E.x + X.x;
+      out[(2 * k + 1) * es_dst] = E.y + X.y;
+    }
+  }
+}
+
+// Inverse: packed half spectra -> real lines (scaled by 1/M here, 1/2 in the split).
+template <bool XMAJOR>
+__global__ void __launch_bounds__(FFT_THREADS)
+    k_rfft_inv(const double* src, double* dst, int N, int logm,
+               int nl, int groups_per_plane, long long plane_src, long long plane_dst,
+               long long gstride_src, long long gstride_dst, long long es_src, long long ls_src,
+               long long es_dst, long long ls_dst, long long off_dst,
+               const double2* __restrict__ tw) {
+  extern __shared__ double2 sm[];
+  const int M = N >> 1;
+  const int kp = blockIdx.x / groups_per_plane, gp = blockIdx.x - kp * groups_per_plane;
+  const long long bs = kp * plane_src + gp * gstride_src;
+  const long long bd = off_dst + kp * plane_dst + gp * gstride_dst;
+  // Z_k = E_k + i O_k, E_k = (X_k + conj X_M-k) / 2, O_k = (X_k - conj X_M-k) conj(W^k) / 2
+  for (int q = threadIdx.x; q < nl * M; q += blockDim.x) {
+    const int l = XMAJOR ? q / M : q % nl, k = XMAJOR ? q - (q / M) * M : q / nl;
+    const double* in = src + bs + l * ls_src;
+    double2 Z;
+    if (k == 0) {
+      const double x0 = in[0], xm = in[es_src];
+      Z = make_double2(0.5 * (x0 + xm), 0.5 * (x0 - xm));
+    } else {
+      const double2 a = make_double2(in[(2 * k) * es_src], in[(2 * k + 1) * es_src]);
+      const double2 b = make_double2(in[(2 * (M - k)) * es_src], in[(2 * (M - k) + 1) * es_src]);
+      const double2 E = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+      const double2 D = make_double2(0.5 * (a.x - b.x), 0.5 * (a.y + b.y));
+      const double2 wc = make_double2(tw[k].x, -tw[k].y);
+      const double2 O = cmul(D, wc);
+      Z = make_double2(E.x - O.y, E.y + O.x);
+    }
+    sm[l * M + brev(k, logm)] = Z;
+  }
+  fft_stages<true>(sm, nl, M, N, tw);
+  const double sc = 1.0 / (double)M;
+  const double* smd = reinterpret_cast<const double*>(sm);
+  for (int q = threadIdx.x; q < nl * N; q += blockDim.x) {
+    const int l = XMAJOR ? q / N : q % nl, e = XMAJOR ? q - (q / N) * N : q / nl;
+    dst[bd + l * ls_dst + e * es_dst] = smd[2 * (l * M + (e >> 1)) + (e & 1)] * sc;
+  }
+}
+
+// ---- Gaussian elimination in k (solver.f90:74-129) ------------------------------------------
+// Wavenumber of packed position p of a length-n line: 0, n/2, or p/2.
+__host__ __device__ __forceinline__ int wavenumber(int p, int n) {
+  return p == 0 ? 0 : (p == 1 ? n / 2 : p >> 1);
+}
+
+// Reciprocal pivots per (wavenumber pair, k), built once: ztab[k][ky][kx] = z of
+// solver.f90:80,89 (k < kmax), and at k = kmax the raw pivot bb - a d (:103-107).
+__global__ void __launch_bounds__(256) k_ztab(int nkx, int nky, int ktot, int ne,
+                                              const double* __restrict__ xrt,
+                                              const double* __restrict__ yrt,
+                                              const double* __restrict__ a,
+                                              const double* __restrict__ b,
+                                              const double* __restrict__ c,
+                                              double* __restrict__ ztab) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nkx * nky) return;
+  const int kx = (int)(t % nkx), ky = (int)(t / nkx);
+  const double xy = xrt[kx] + yrt[ky];
+  const long long st = (long long)nkx * nky;
+  double d = 0.0;
+  for (int k = 0; k < ktot - 1; ++k) {
+    const double bb = b[k] + xy;
+    const double z = 1.0 / (bb - a[k] * d);
+    ztab[k * st + t] = z;
+    d = c[k] * z;
+  }
+  const double bb = (ne == 3) ? b[ktot - 1] : b[ktot - 1] + xy;
+  ztab[(long long)(ktot - 1) * st + t] = bb - a[ktot - 1] * d;
+}
+
+// One lane per (i, j) mode of the compact spectral array.  LDS: the forward sweep's values,
+// TRI_MODES x ktot doubles (else they go back to the array and are read again).
+template <bool LDS>
+__global__ void __launch_bounds__(TRI_MODES)
+    k_tri(double* __restrict__ p, int itot, int jtot, int ktot, const double* __restrict__ a,
+          const double* __restrict__ c, const double* __restrict__ ztab) {
+  extern __shared__ double fw[];
+  const long long plane = (long long)itot * jtot;
+  const long long m = (long long)blockIdx.x * TRI_MODES + threadIdx.x;
+  if (m >= plane) return;
+  const int i = (int)(m % itot), j = (int)(m / itot);
+  const int nkx = itot / 2 + 1, nky = jtot / 2 + 1;
+  const long long zb = (long long)wavenumber(j, jtot) * nkx + wavenumber(i, itot);
+  const long long zs = (long long)nkx * nky;
+  double prev = 0.0;
+  for (int k = 0; k < ktot - 1; ++k) {
+    const double z = ztab[k * zs + zb];
+    prev = (p[k * plane + m] - a[k] * prev) * z;
+    if (LDS)
+      fw[k * TRI_MODES + threadIdx.x] = prev;
+    else
+      p[k * plane + m] = prev;
+  }
+  const double zl = ztab[(long long)(ktot - 1) * zs + zb];
+  double nxt = zl != 0.0 ? (p[(long long)(ktot - 1) * plane + m] - a[ktot - 1] * prev) / zl : 0.0;
+  p[(long long)(ktot - 1) * plane + m] = nxt;
+  for (int k = ktot - 2; k >= 0; --k) {
+    const double d = c[k] * ztab[k * zs + zb];
+    const double v = LDS ? fw[k * TRI_MODES + threadIdx.x] : p[k * plane + m];
+    nxt = v - d * nxt;
+    p[k * plane + m] = nxt;
+  }
+}
+
+// ---- stencils --------------------------------------------------------------------------------
+// mom.f90:9-50 (momxad), at interior cell (i, j, k)
+__device__ __forceinline__ double momxad(const FGrid& g, const double* Uo, const double* Vo,
+                                         const double* Wo, const double* Uoo, const double* Voo,
+                                         const double* Woo, int i, int j, int k) {
+  const int ip = i + 1, jp = j + 1, kp = k + 1, im = i - 1, jm = j - 1, km = k - 1;
+#define F(A, a, b, c) A[fidx(g, a, b, c)]
+  const double uuip = 0.25 * (F(Uo, ip, j, k) + F(Uo, i, j, k)) * (F(Uo, ip, j, k) + F(Uo, i, j, k));
+  const double uuim = 0.25 * (F(Uo, im, j, k) + F(Uo, i, j, k)) * (F(Uo, im, j, k) + F(Uo, i, j, k));
+  const double uvjp = 0.25 * (F(Uo, i, jp, k) + F(Uo, i, j, k)) * (F(Vo, ip, j, k) + F(Vo, i, j, k));
+  const double uvjm = 0.25 * (F(Uo, i, jm, k) + F(Uo, i, j, k)) * (F(Vo, ip, jm, k) + F(Vo, i, jm, k));
+  const double uwkp = 0.25 * (F(Uo, i, j, kp) + F(Uo, i, j, k)) * (F(Wo, ip, j, k) + F(Wo, i, j, k));
+  const double uwkm = 0.25 * (F(Uo, i, j, km) + F(Uo, i, j, k)) * (F(Wo, ip, j, km) + F(Wo, i, j, km));
+  const double uuipo = 0.25 * (F(Uoo, ip, j, k) + F(Uoo, i, j, k)) * (F(Uoo, ip, j, k) + F(Uoo, i, j, k));
+  const double uuimo = 0.25 * (F(Uoo, im, j, k) + F(Uoo, i, j, k)) * (F(Uoo, im, j, k) + F(Uoo, i, j, k));
+  const double uvjpo = 0.25 * (F(Uoo, i, jp, k) + F(Uoo, i, j, k)) * (F(Voo, ip, j, k) + F(Voo, i, j, k));
+  const double uvjmo = 0.25 * (F(Uoo, i, jm, k) + F(Uoo, i, j, k)) * (F(Voo, ip, jm, k) + F(Voo, i, jm, k));
+  const double uwkpo = 0.25 * (F(Uoo, i, j, kp) + F(Uoo, i, j, k)) * (F(Woo, ip, j, k) + F(Woo, i, j, k));
+  const double uwkmo = 0.25 * (F(Uoo, i, j, km) + F(Uoo, i, j, k)) * (F(Woo, ip, j, km) + F(Woo, i, j, km));
+  return g.dxi * ((1.5 * (-uuip + uuim)) - (.5 * (-uuipo + uuimo))) +
+         g.dyi * ((1.5 * (-uvjp + uvjm)) - (.5 * (-uvjpo + uvjmo))) +
+         g.dzi * ((1.5 * (-uwkp + uwkm)) - (.5 * (-uwkpo + uwkmo)));
+}
+
+// mom.f90:158-199 (momyad)
+__device__ __forceinline__ double momyad(const FGrid& g, const double* Uo, const double* Vo,
+                                         const double* Wo, const double* Uoo, const double* Voo,
+                                         const double* Woo, int i, int j, int k) {
+  const int ip = i + 1, jp = j + 1, kp = k + 1, im = i - 1, jm = j - 1, km = k - 1;
+  const double uvip = 0.25 * (F(Uo, i, j, k) + F(Uo, i, jp, k)) * (F(Vo, i, j, k) + F(Vo, ip, j, k));
+  const double uvim = 0.25 * (F(Uo, im, j, k) + F(Uo, im, jp, k)) * (F(Vo, i, j, k) + F(Vo, im, j, k));
+  const double vvjp = 0.25 * (F(Vo, i, j, k) + F(Vo, i, jp, k)) * (F(Vo, i, j, k) + F(Vo, i, jp, k));
+  const double vvjm = 0.25 * (F(Vo, i, j, k) + F(Vo, i, jm, k)) * (F(Vo, i, j, k) + F(Vo, i, jm, k));
+  const double wvkp = 0.25 * (F(Wo, i, j, k) + F(Wo, i, jp, k)) * (F(Vo, i, j, kp) + F(Vo, i, j, k));
+  const double wvkm = 0.25 * (F(Wo, i, j, km) + F(Wo, i, jp, km)) * (F(Vo, i, j, km) + F(Vo, i, j, k));
+  const double uvipo = 0.25 * (F(Uoo, i, j, k) + F(Uoo, i, jp, k)) * (F(Voo, i, j, k) + F(Voo, ip, j, k));
+  const double uvimo = 0.25 * (F(Uoo, im, j, k) + F(Uoo, im, jp, k)) * (F(Voo, i, j, k) + F(Voo, im, j, k));
+  const double vvjpo = 0.25 * (F(Voo, i, j, k) + F(Voo, i, jp, k)) * (F(Voo, i, j, k) + F(Voo, i, jp, k));
+  const double vvjmo = 0.25 * (F(Voo, i, j, k) + F(Voo, i, jm, k)) * (F(Voo, i, j, k) + F(Voo, i, jm, k));
+  const double wvkpo = 0.25 * (F(Woo, i, j, k) + F(Woo, i, jp, k)) * (F(Voo, i, j, kp) + F(Voo, i, j, k));
+  const double wvkmo = 0.25 * (F(Woo, i, j, km) + F(Woo, i, jp, km)) * (F(Voo, i, j, km) + F(Voo, i, j, k));
+  return g.dxi * ((1.5 * (-uvip + uvim)) - (.5 * (-uvipo + uvimo))) +
+         g.dyi * ((1.5 * (-vvjp + vvjm)) - (.5 * (-vvjpo + vvjmo))) +
+         g.dzi * ((1.5 * (-wvkp + wvkm)) - (.5 * (-wvkpo + wvkmo)));
+}
+
+// mom.f90:286-327 (momzad)
+__device__ __forceinline__ double momzad(const FGrid& g, const double* Uo, const double* Vo,
+                                         const double* Wo, const double* Uoo, const double* Voo,
+                                         const double* Woo, int i, int j, int k) {
+  const int ip = i + 1, jp = j + 1, kp = k + 1, im = i - 1, jm = j - 1, km = k - 1;
+  const double uwip = 0.25 * (F(Wo, i, j, k) + F(Wo, ip, j, k)) * (F(Uo, i, j, k) + F(Uo, i, j, kp));
+  const double uwim = 0.25 * (F(Wo, i, j, k) + F(Wo, im, j, k)) * (F(Uo, im, j, k) + F(Uo, im, j, kp));
+  const double vwjp = 0.25 * (F(Wo, i, j, k) + F(Wo, i, jp, k)) * (F(Vo, i, j, k) + F(Vo, i, j, kp));
+  const double vwjm = 0.25 * (F(Wo, i, j, k) + F(Wo, i, jm, k)) * (F(Vo, i, jm, k) + F(Vo, i, jm, kp));
+  const double wwkp = 0.25 * (F(Wo, i, j, k) + F(Wo, i, j, kp)) * (F(Wo, i, j, k) + F(Wo, i, j, kp));
+  const double wwkm = 0.25 * (F(Wo, i, j, k) + F(Wo, i, j, km)) * (F(Wo, i, j, k) + F(Wo, i, j, km));
+  const double uwipo = 0.25 * (F(Woo, i, j, k) + F(Woo, ip, j, k)) * (F(Uoo, i, j, k) + F(Uoo, i, j, kp));
+  const double uwimo = 0.25 * (F(Woo, i, j, k) + F(Woo, im, j, k)) * (F(Uoo, im, j, k) + F(Uoo, im, j, kp));
+  const double vwjpo = 0.25 * (F(Woo, i, j, k) + F(Woo, i, jp, k)) * (F(Voo, i, j, k) + F(Voo, i, j, kp));
+  const double vwjmo = 0.25 * (F(Woo, i, j, k) + F(Woo, i, jm, k)) * (F(Voo, i, jm, k) + F(Voo, i, jm, kp));
+  const double wwkpo = 0.25 * (F(Woo, i, j, k) + F(Woo, i, j, kp)) * (F(Woo, i, j, k) + F(Woo, i, j, kp));
+  const double wwkmo = 0.25 * (F(Woo, i, j, k) + F(Woo, i, j, km)) * (F(Woo, i, j, k) + F(Woo, i, j, km));
+  return g.dxi * ((1.5 * (-uwip + uwim)) - (.5 * (-uwipo + uwimo))) +
+         g.dyi * ((1.5 * (-vwjp + vwjm)) - (.5 * (-vwjpo + vwjmo))) +
+         g.dzi * ((1.5 * (-wwkp + wwkm)) - (.5 * (-wwkpo + wwkmo)));
+}
+
+// fftuvw.f90:30-105: the three right-hand sides of one interior cell, scaled by Rep, written
+// to the compact arrays (dwdt(:,:,kmax) = 0)
+__global__ void __launch_bounds__(256)
+    k_mom_rhs(FGrid g, double rep, double dt, const double* __restrict__ uo,
+              const double* __restrict__ vo, const double* __restrict__ wo,
+              const double* __restrict__ uoo, const double* __restrict__ voo,
+              const double* __restrict__ woo, const double* __restrict__ p,
+              const double* __restrict__ fx, const double* __restrict__ fy,
+              const double* __restrict__ fz, double* __restrict__ cu, double* __restrict__ cv,
+              double* __restrict__ cw) {
+  const long long n = (long long)g.itot * g.jtot * g.ktot;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.itot) + 1;
+  const long long r = t / g.itot;
+  const int j = (int)(r % g.jtot) + 1, k = (int)(r / g.jtot) + 1;
+  const long long q = fidx(g, i, j, k);
+  const double durk = (-g.dxi * ((F(p, i + 1, j, k) - F(p, i, j, k)))) + fx[q];
+  const double dvrk = (-g.dyi * (F(p, i, j + 1, k) - F(p, i, j, k))) + fy[q];
+  const double dwrk = (-g.dzi * (F(p, i, j, k + 1) - F(p, i, j, k))) + fz[q];
+  double du = -durk, dv = -dvrk, dw = -dwrk;
+  du = du - momxad(g, uo, vo, wo, uoo, voo, woo, i, j, k);
+  dv = dv - momyad(g, uo, vo, wo, uoo, voo, woo, i, j, k);
+  dw = dw - momzad(g, uo, vo, wo, uoo, voo, woo, i, j, k);
+  du = du - (uo[q] / dt);
+  dv = dv - (vo[q] / dt);
+  dw = dw - (wo[q] / dt);
+  cu[t] = du * rep;
+  cv[t] = dv * rep;
+  cw[t] = (k == g.ktot) ? 0.0 : dw * rep;
+}
+#undef F
+
+// fillps.f90:8-47 into the compact array
+__global__ void __launch_bounds__(256) k_fillps(FGrid g, double dt, const double* __restrict__ u,
+                                                const double* __restrict__ v,
+                                                const double* __restrict__ w,
+                                                double* __restrict__ cp) {
+  const long long n = (long long)g.itot * g.jtot * g.ktot;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.itot) + 1;
+  const long long r = t / g.itot;
+  const int j = (int)(r % g.jtot) + 1, k = (int)(r / g.jtot) + 1;
+  const double dti = 1.0 / dt;
+  const double dtidxi = dti * g.dxi, dtidyi = dti * g.dyi, dtidzi = dti * g.dzi;
+  const long long q = fidx(g, i, j, k);
+  cp[t] = ((w[q] - w[q - g.sz]) * dtidzi + (v[q] - v[q - g.sy]) * dtidyi +
+           (u[q] - u[q - 1]) * dtidxi);
+}
+
+// a compact array into the interior of a field
+__global__ void __launch_bounds__(256) k_to_field(FGrid g, const double* __restrict__ c,
+                                                  double* __restrict__ f) {
+  const long long n = (long long)g.itot * g.jtot * g.ktot;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.itot) + 1;
+  const long long r = t / g.itot;
+  f[fidx(g, i, (int)(r % g.jtot) + 1, (int)(r / g.jtot) + 1)] = c[t];
+}
+
+// correc.f90:11-38
+__global__ void __launch_bounds__(256) k_correc(FGrid g, double dt, double* __restrict__ u,
+                                                double* __restrict__ v, double* __restrict__ w,
+                                                const double* __restrict__ p) {
+  const long long n = (long long)g.itot * g.jtot * g.ktot;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.itot) + 1;
+  const long long r = t / g.itot;
+  const int j = (int)(r % g.jtot) + 1, k = (int)(r / g.jtot) + 1;
+  const double factori = dt * g.dxi, factorj = dt * g.dyi, factork = dt * g.dzi;
+  const long long q = fidx(g, i, j, k);
+  u[q] = u[q] - factori * (p[q + 1] - p[q]);
+  v[q] = v[q] - factorj * (p[q + g.sy] - p[q]);
+  w[q] = w[q] - factork * (p[q + g.sz] - p[q]);
+}
+
+// pnew += p on the interior (main.f90:406-408; boundp then rewrites every halo and ghost)
+__global__ void __launch_bounds__(256) k_addp(FGrid g, double* __restrict__ pn,
+                                              const double* __restrict__ p) {
+  const long long n = (long long)g.itot * g.jtot * g.ktot;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.itot) + 1;
+  const long long r = t / g.itot;
+  const long long q = fidx(g, i, (int)(r % g.jtot) + 1, (int)(r / g.jtot) + 1);
+  pn[q] = pn[q] + p[q];
+}
+
+// Final value of a wall ghost / top face after the k step of bounduvw (bound.f90:38-56) or
+// boundp (:93-98), at interior column (i, j).  kind 0: u or v (ghost = -interior), 1: w
+// (0 on both wall faces, k1 copies kmax-1), 2: p (ghost = interior).
+__device__ __forceinline__ double kstep(const FGrid& g, const double* f, int kind, int i, int j,
+                                        int k) {
+  const int kmax = g.ktot, k1 = g.ktot + 1;
+  if (kind == 0) {
+    if (k == 0) return -f[fidx(g, i, j, 1)];
+    if (k == k1) return -f[fidx(g, i, j, kmax)];
+  } else if (kind == 1) {
+    if (k == 0 || k == kmax) return 0.0;
+    if (k == k1) return f[fidx(g, i, j, kmax - 1)];
+  } else {
+    if (k == 0) return f[fidx(g, i, j, 1)];
+    if (k == k1) return f[fidx(g, i, j, kmax)];
+  }
+  return f[fidx(g, i, j, k)];
+}
+
+// bounduvw / boundp in one launch: the k step, then updthalos in x, then in y
+// (bound.f90:61-76, 100-104; bound.f90:141-190).  Every halo or ghost cell takes its final
+// value straight from the interior column the sequence copies it from.  One thread per
+// (column, field); columns are the (i1+1)(j1+1) of the plane.
+__global__ void __launch_bounds__(256) k_bound(FGrid g, double* __restrict__ f0,
+                                               double* __restrict__ f1, double* __restrict__ f2,
+                                               int nfields, int kind0) {
+  const long long ncol = g.sy * (long long)(g.jtot + 2);
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncol * nfields) return;
+  const int fi = (int)(t / ncol);
+  const long long col = t - fi * ncol;
+  const int i = (int)(col % g.sy), j = (int)(col / g.sy);
+  double* f = fi == 0 ? f0 : (fi == 1 ? f1 : f2);
+  const int kind = kind0 == 2 ? 2 : (fi == 2 ? 1 : 0);
+  const int si = i == 0 ? g.itot : (i == g.itot + 1 ? 1 : i);
+  const int sj = j == 0 ? g.jtot : (j == g.jtot + 1 ? 1 : j);
+  const bool inner = si == i && sj == j;
+  if (inner) {
+    f[fidx(g, i, j, 0)] = kstep(g, f, kind, i, j, 0);
+    f[fidx(g, i, j, g.ktot + 1)] = kstep(g, f, kind, i, j, g.ktot + 1);
+    if (kind == 1) f[fidx(g, i, j, g.ktot)] = 0.0;
+  } else {
+    for (int k = 0; k <= g.ktot + 1; ++k) f[fidx(g, i, j, k)] = kstep(g, f, kind, si, sj, k);
+  }
+}
+
+// chkdiv.f90:7-61: per-block sums and maxima of the discrete divergence
+__global__ void __launch_bounds__(256) k_div(FGrid g, const double* __restrict__ u,
+                                             const double* __restrict__ v,
+                                             const double* __restrict__ w,
+                                             double* __restrict__ part) {
+  __shared__ double ss[256], sm[256];
+  const long long n = (long long)g.itot * g.jtot * g.ktot;
+  double s = 0.0, m = 0.0;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % g.itot) + 1;
+    const long long r = t / g.itot;
+    const int j = (int)(r % g.jtot) + 1, k = (int)(r / g.jtot) + 1;
+    const long long q = fidx(g, i, j, k);
+    const double div = (w[q] - w[q - g.sz]) * g.dzi + (v[q] - v[q - g.sy]) * g.dyi +
+                       (u[q] - u[q - 1]) * g.dxi;
+    s = s + div;
+    m = fmax(m, fabs(div));
+  }
+  ss[threadIdx.x] = s, sm[threadIdx.x] = m;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) {
+      ss[threadIdx.x] = ss[threadIdx.x] + ss[threadIdx.x + h];
+      sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + h]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[2 * blockIdx.x] = ss[0], part[2 * blockIdx.x + 1] = sm[0];
+}
+
+}  // namespace
+
+struct fld_ctx {
+  ibx_grid grid{};
+  FGrid g{};
+  double rep = 0, dt = 0;
+  int device = 0;
+  hipStream_t own = nullptr, stream = nullptr;
+  std::string err;
+  long long ncomp = 0;                  // itot * jtot * ktot
+  double* work[3] = {nullptr, nullptr, nullptr};  // compact spectral arrays
+  double* ztab[4] = {nullptr, nullptr, nullptr, nullptr};  // index ne = 1..3
+  double* coef[4] = {nullptr, nullptr, nullptr, nullptr};  // a, c per ne (2 * ktot each)
+  double2* tw_x = nullptr;
+  double2* tw_y = nullptr;
+  double* part = nullptr;  // chkdiv partials
+  int logmx = 0, logmy = 0;
+  bool tri_lds = true;
+  size_t lds_y = 0;
+  int ylines = Y_LINES, xlines = 1;
+};
+
+namespace {
+
+int fail(fld_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPF(ctx, call)                                                                      \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return fail((ctx), FLD_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_) +     \
+                                        " (" + __FILE__ + ":" + std::to_string(__LINE__) + ")"); \
+  } while (0)
+
+inline unsigned nblocks(long long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+int ilog2(int n) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
+
+bool pow2(int n) { return n >= 2 && (n & (n - 1)) == 0; }
+
+// exp(-2 pi i k / n), k < n/2, from long-double sine and cosine
+std::vector<double2> twiddles(int n) {
+  std::vector<double2> t((size_t)std::max(1, n / 2));
+  const long double pi = 3.141592653589793238462643383279502884L;
+  for (int k = 0; k < n / 2; ++k) {
+    const long double a = 2.0L * pi * (long double)k / (long double)n;
+    t[(size_t)k] = make_double2((double)cosl(a), (double)(-sinl(a)));
+  }
+  return t;
+}
+
+// initsolver.f90:108-132 by wavenumber: 0, 1 .. n/2 - 1, and n/2
+std::vector<double> eigen(int n, double di) {
+  std::vector<double> e((size_t)(n / 2 + 1));
+  const double pi = std::acos(-1.0);
+  e[0] = 0.0;
+  for (int m = 1; m < n / 2; ++m) {
+    const double s = std::sin((double)(2 * m) * pi / (2.0 * n));
+    e[(size_t)m] = -(4.0 * di * di * (s * s));
+  }
+  e[(size_t)(n / 2)] = -(4.0 * di * di);
+  return e;
+}
+
+// initsolver.f90:23-99 for e = 1, 2, 3 (main mesh)
+void tridiag(const fld_ctx* c, int e, std::vector<double>& a, std::vector<double>& b,
+             std::vector<double>& cc) {
+  const int kmax = c->grid.ktot;
+  const double dzi = c->grid.dzi;
+  a.assign((size_t)kmax, 0.0), b.assign((size_t)kmax, 0.0), cc.assign((size_t)kmax, 0.0);
+  for (int k = 0; k < kmax; ++k) {
+    a[(size_t)k] = dzi * dzi;
+    cc[(size_t)k] = dzi * dzi;
+    b[(size_t)k] = -(a[(size_t)k] + cc[(size_t)k]);
+  }
+  if (e == 1) {
+    b[0] = b[0] + a[0];
+    b[(size_t)kmax - 1] = b[(size_t)kmax - 1] + cc[(size_t)kmax - 1];
+    a[0] = 0.0;
+    cc[(size_t)kmax - 1] = 0.0;
+  } else {
+    if (e == 2) {
+      b[0] = b[0] - a[0];
+      b[(size_t)kmax - 1] = b[(size_t)kmax - 1] - cc[(size_t)kmax - 1];
+      a[0] = 0.0;
+      cc[(size_t)kmax - 1] = 0.0;
+    } else {
+      b[(size_t)kmax - 1] = 1.0;
+      a[0] = 0.0;
+      a[(size_t)kmax - 1] = 0.0;
+      cc[(size_t)kmax - 1] = 0.0;
+    }
+    for (int k = 0; k < kmax; ++k) b[(size_t)k] = b[(size_t)k] - (c->rep / c->dt);
+    if (e == 3) b[(size_t)kmax - 1] = 1.0;
+  }
+}
+
+// x transform of the interior lines of a field (or of a compact array), forward or inverse
+int fft_x(fld_ctx* c, bool inv, const double* src, bool src_field, double* dst, bool dst_field) {
+  const FGrid& g = c->g;
+  const int N = g.itot, nl = c->xlines;
+  const int gpp = g.jtot / nl;  // workgroups per k plane
+  const long long cp = (long long)g.itot * g.jtot;
+  const long long row_s = src_field ? g.sy : g.itot, plane_s = src_field ? g.sz : cp;
+  const long long row_d = dst_field ? g.sy : g.itot, plane_d = dst_field ? g.sz : cp;
+  const long long off = 1 + g.sy + g.sz;  // (1, 1, 1)
+  const size_t lds = (size_t)nl * N * sizeof(double);
+  const unsigned nb = (unsigned)(gpp * g.ktot);
+  if (!inv)
+    k_rfft_fwd<true><<<nb, FFT_THREADS, lds, c->stream>>>(
+        src, dst, N, c->logmx, nl, gpp, plane_s, plane_d, nl * row_s, nl * row_d, 1, row_s, 1,
+        row_d, src_field ? off : 0, c->tw_x);
+  else
+    k_rfft_inv<true><<<nb, FFT_THREADS, lds, c->stream>>>(
+        src, dst, N, c->logmx, nl, gpp, plane_s, plane_d, nl * row_s, nl * row_d, 1, row_s, 1,
+        row_d, dst_field ? off : 0, c->tw_x);
+  HIPF(c, hipGetLastError());
+  return FLD_OK;
+}
+
+// y transform in place on a compact array
+int fft_y(fld_ctx* c, bool inv, double* a) {
+  const FGrid& g = c->g;
+  const int N = g.jtot, nl = c->ylines;
+  const int gpp = g.itot / nl;
+  const long long cp = (long long)g.itot * g.jtot;
+  const unsigned nb = (unsigned)(gpp * g.ktot);
+  if (!inv)
+    k_rfft_fwd<false><<<nb, FFT_THREADS, c->lds_y, c->stream>>>(
+        a, a, N, c->logmy, nl, gpp, cp, cp, nl, nl, g.itot, 1, g.itot, 1, 0, c->tw_y);
+  else
+    k_rfft_inv<false><<<nb, FFT_THREADS, c->lds_y, c->stream>>>(
+        a, a, N, c->logmy, nl, gpp, cp, cp, nl, nl, g.itot, 1, g.itot, 1, 0, c->tw_y);
+  HIPF(c, hipGetLastError());
+  return FLD_OK;
+}
+
+int tri(fld_ctx* c, double* a, int ne) {
+  const FGrid& g = c->g;
+  const long long modes = (long long)g.itot * g.jtot;
+  const double* ac = c->coef[ne];
+  const double* cc = c->coef[ne] + g.ktot;
+  if (c->tri_lds)
+    k_tri<true><<<nblocks(modes, TRI_MODES), TRI_MODES, (size_t)TRI_MODES * g.ktot * sizeof(double),
+                  c->stream>>>(a, g.itot, g.jtot, g.ktot, ac, cc, c->ztab[ne]);
+  else
+    k_tri<false><<<nblocks(modes, TRI_MODES), TRI_MODES, 0, c->stream>>>(a, g.itot, g.jtot, g.ktot,
+                                                                          ac, cc, c->ztab[ne]);
+  HIPF(c, hipGetLastError());
+  return FLD_OK;
+}
+
+// solver1d on a compact right-hand side, result into the interior of `out`
+int solve_compact(fld_ctx* c, double* a, int ne, double* out) {
+  int rc;
+  if ((rc = fft_x(c, false, a, false, a, false)) || (rc = fft_y(c, false, a)) ||
+      (rc = tri(c, a, ne)) || (rc = fft_y(c, true, a)) || (rc = fft_x(c, true, a, false, out, true)))
+    return rc;
+  return FLD_OK;
+}
+
+int bound(fld_ctx* c, double* f0, double* f1, double* f2, int nfields, int kind0) {
+  const long long ncol = c->g.sy * (long long)(c->g.jtot + 2);
+  k_bound<<<nblocks(ncol * nfields, 256), 256, 0, c->stream>>>(c->g, f0, f1, f2, nfields, kind0);
+  HIPF(c, hipGetLastError());
+  return FLD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fld_create(fld_ctx** out, const ibx_grid* gr, double rep, double dt, int device) {
+  if (!out || !gr) return FLD_E_ARG;
+  *out = nullptr;
+  if (!pow2(gr->itot) || !pow2(gr->jtot) || gr->itot < 4 || gr->jtot < 4 || gr->ktot < 2)
+    return FLD_E_ARG;
+  // one line of each transform must fit in LDS
+  if ((size_t)gr->itot * sizeof(double) > LDS_MAX || (size_t)gr->jtot * sizeof(double) > LDS_MAX)
+    return FLD_E_ARG;
+  if (!(rep > 0 && dt > 0 && gr->dxi > 0 && gr->dyi > 0 && gr->dzi > 0)) return FLD_E_ARG;
+  fld_ctx* c = new fld_ctx();
+  c->grid = *gr;
+  c->rep = rep, c->dt = dt, c->device = device;
+  FGrid& g = c->g;
+  g.itot = gr->itot, g.jtot = gr->jtot, g.ktot = gr->ktot;
+  g.sy = gr->itot + 2;
+  g.sz = (long long)(gr->itot + 2) * (gr->jtot + 2);
+  g.dxi = gr->dxi, g.dyi = gr->dyi, g.dzi = gr->dzi;
+  c->ncomp = (long long)g.itot * g.jtot * g.ktot;
+  c->logmx = ilog2(g.itot / 2), c->logmy = ilog2(g.jtot / 2);
+  c->xlines = std::min(g.jtot, std::max(1, 2048 / g.itot));
+  c->ylines = std::min(g.itot, Y_LINES);
+  while (c->ylines > 1 && (size_t)c->ylines * g.jtot * sizeof(double) > LDS_MAX) c->ylines >>= 1;
+  c->lds_y = (size_t)c->ylines * g.jtot * sizeof(double);
+  c->tri_lds = (size_t)TRI_MODES * g.ktot * sizeof(double) <= LDS_MAX;
+  auto bail = [&](int code, const char* what) {
+    fld_destroy(c);
+    (void)what;
+    return code;
+  };
+  if (hipSetDevice(device) != hipSuccess) return bail(FLD_E_HIP, "device");
+  if (hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking) != hipSuccess)
+    return bail(FLD_E_HIP, "stream");
+  c->stream = c->own;
+  for (auto& w : c->work)
+    if (hipMalloc(&w, (size_t)c->ncomp * sizeof(double)) != hipSuccess) return bail(FLD_E_HIP, "work");
+  // dynamic LDS above the default limit for the y transform and the elimination
+  if (hipFuncSetAttribute((const void*)k_rfft_fwd<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)LDS_MAX) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_rfft_inv<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)LDS_MAX) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_rfft_fwd<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)LDS_MAX) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_rfft_inv<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)LDS_MAX) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_tri<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)LDS_MAX) != hipSuccess)
+    return bail(FLD_E_HIP, "lds");
+  // twiddles, eigenvalues, coefficients and the pivot tables of e = 1, 2, 3
+  const std::vector<double2> twx = twiddles(g.itot), twy = twiddles(g.jtot);
+  const std::vector<double> xrt = eigen(g.itot, g.dxi), yrt = eigen(g.jtot, g.dyi);
+  double *dx = nullptr, *dy = nullptr, *dabc = nullptr;
+  bool ok = hipMalloc(&c->tw_x, twx.size() * sizeof(double2)) == hipSuccess &&
+            hipMalloc(&c->tw_y, twy.size() * sizeof(double2)) == hipSuccess &&
+            hipMalloc(&dx, xrt.size() * sizeof(double)) == hipSuccess &&
+            hipMalloc(&dy, yrt.size() * sizeof(double)) == hipSuccess &&
+            hipMalloc(&dabc, 3 * (size_t)g.ktot * sizeof(double)) == hipSuccess &&
+            hipMalloc(&c->part, 2 * 1024 * sizeof(double)) == hipSuccess;
+  ok = ok && hipMemcpy(c->tw_x, twx.data(), twx.size() * sizeof(double2), hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(c->tw_y, twy.data(), twy.size() * sizeof(double2), hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(dx, xrt.data(), xrt.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(dy, yrt.data(), yrt.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess;
+  const int nkx = g.itot / 2 + 1, nky = g.jtot / 2 + 1;
+  for (int ne = 1; ok && ne <= 3; ++ne) {
+    std::vector<double> a, b, cc;
+    tridiag(c, ne, a, b, cc);
+    std::vector<double> abc(a);
+    abc.insert(abc.end(), b.begin(), b.end());
+    abc.insert(abc.end(), cc.begin(), cc.end());
+    std::vector<double> ac(a);
+    ac.insert(ac.end(), cc.begin(), cc.end());
+    ok = hipMalloc(&c->coef[ne], 2 * (size_t)g.ktot * sizeof(double)) == hipSuccess &&
+         hipMemcpy(c->coef[ne], ac.data(), ac.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(dabc, abc.data(), abc.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMalloc(&c->ztab[ne], (size_t)nkx * nky * g.ktot * sizeof(double)) == hipSuccess;
+    if (ok) {
+      k_ztab<<<nblocks((long long)nkx * nky, 256), 256, 0, c->own>>>(
+          nkx, nky, g.ktot, ne, dx, dy, dabc, dabc + g.ktot, dabc + 2 * g.ktot, c->ztab[ne]);
+      ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(c->own) == hipSuccess;
+    }
+  }
+  if (dx) (void)hipFree(dx);
+  if (dy) (void)hipFree(dy);
+  if (dabc) (void)hipFree(dabc);
+  if (!ok) return bail(FLD_E_HIP, "tables");
+  *out = c;
+  return FLD_OK;
+}
+
+void fld_destroy(fld_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& w : c->work)
+    if (w) (void)hipFree(w);
+  for (int ne = 1; ne <= 3; ++ne) {
+    if (c->ztab[ne]) (void)hipFree(c->ztab[ne]);
+    if (c->coef[ne]) (void)hipFree(c->coef[ne]);
+  }
+  if (c->tw_x) (void)hipFree(c->tw_x);
+  if (c->tw_y) (void)hipFree(c->tw_y);
+  if (c->part) (void)hipFree(c->part);
+  if (c->own) (void)hipStreamDestroy(c->own);
+  delete c;
+}
+
+const char* fld_last_error(const fld_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int fld_set_stream(fld_ctx* c, void* s) {
+  if (!c) return FLD_E_ARG;
+  c->stream = s ? (hipStream_t)s : c->own;
+  return FLD_OK;
+}
+
+int fld_sync(fld_ctx* c) {
+  if (!c) return FLD_E_ARG;
+  HIPF(c, hipSetDevice(c->device));
+  HIPF(c, hipStreamSynchronize(c->stream));
+  return FLD_OK;
+}
+
+int fld_solver1d_dev(fld_ctx* c, double* p, int ne) {
+  if (!c) return FLD_E_ARG;
+  if (!p) return fail(c, FLD_E_ARG, "null field");
+  if (ne < 1 || ne > 3) return fail(c, FLD_E_ARG, "ne must be 1, 2 or 3");
+  HIPF(c, hipSetDevice(c->device));
+  int rc = fft_x(c, false, p, true, c->work[0], false);
+  if (rc) return rc;
+  if ((rc = fft_y(c, false, c->work[0])) || (rc = tri(c, c->work[0], ne)) ||
+      (rc = fft_y(c, true, c->work[0])) || (rc = fft_x(c, true, c->work[0], false, p, true)))
+    return rc;
+  return FLD_OK;
+}
+
+int fld_fftuvw_dev(fld_ctx* c, double* unew, double* vnew, double* wnew, const double* uo,
+                   const double* vo, const double* wo, const double* uoo, const double* voo,
+                   const double* woo, const double* pnew, const double* fx, const double* fy,
+                   const double* fz) {
+  if (!c) return FLD_E_ARG;
+  if (!(unew && vnew && wnew && uo && vo && wo && uoo && voo && woo && pnew && fx && fy && fz))
+    return fail(c, FLD_E_ARG, "null field");
+  HIPF(c, hipSetDevice(c->device));
+  k_mom_rhs<<<nblocks(c->ncomp, 256), 256, 0, c->stream>>>(c->g, c->rep, c->dt, uo, vo, wo, uoo,
+                                                           voo, woo, pnew, fx, fy, fz, c->work[0],
+                                                           c->work[1], c->work[2]);
+  HIPF(c, hipGetLastError());
+  int rc;
+  if ((rc = solve_compact(c, c->work[2], 3, wnew)) || (rc = solve_compact(c, c->work[0], 2, unew)) ||
+      (rc = solve_compact(c, c->work[1], 2, vnew)))
+    return rc;
+  return FLD_OK;
+}
+
+int fld_bounduvw_dev(fld_ctx* c, double* u, double* v, double* w) {
+  if (!c) return FLD_E_ARG;
+  if (!(u && v && w)) return fail(c, FLD_E_ARG, "null field");
+  HIPF(c, hipSetDevice(c->device));
+  return bound(c, u, v, w, 3, 0);
+}
+
+int fld_boundp_dev(fld_ctx* c, double* p) {
+  if (!c) return FLD_E_ARG;
+  if (!p) return fail(c, FLD_E_ARG, "null field");
+  HIPF(c, hipSetDevice(c->device));
+  return bound(c, p, nullptr, nullptr, 1, 2);
+}
+
+int fld_fillps_dev(fld_ctx* c, const double* u, const double* v, const double* w, double* p) {
+  if (!c) return FLD_E_ARG;
+  if (!(u && v && w && p)) return fail(c, FLD_E_ARG, "null field");
+  HIPF(c, hipSetDevice(c->device));
+  k_fillps<<<nblocks(c->ncomp, 256), 256, 0, c->stream>>>(c->g, c->dt, u, v, w, c->work[0]);
+  k_to_field<<<nblocks(c->ncomp, 256), 256, 0, c->stream>>>(c->g, c->work[0], p);
+  HIPF(c, hipGetLastError());
+  return FLD_OK;
+}
+
+int fld_correc_dev(fld_ctx* c, double* u, double* v, double* w, const double* p) {
+  if (!c) return FLD_E_ARG;
+  if (!(u && v && w && p)) return fail(c, FLD_E_ARG, "null field");
+  HIPF(c, hipSetDevice(c->device));
+  k_correc<<<nblocks(c->ncomp, 256), 256, 0, c->stream>>>(c->g, c->dt, u, v, w, p);
+  HIPF(c, hipGetLastError());
+  return FLD_OK;
+}
+
+int fld_step_dev(fld_ctx* c, double* unew, double* vnew, double* wnew, double* pnew, double* p,
+                 const double* uo, const double* vo, const double* wo, const double* uoo,
+                 const double* voo, const double* woo, const double* fx, const double* fy,
+                 const double* fz) {
+  if (!c) return FLD_E_ARG;
+  if (!(pnew && p)) return fail(c, FLD_E_ARG, "null field");
+  int rc = fld_fftuvw_dev(c, unew, vnew, wnew, uo, vo, wo, uoo, voo, woo, pnew, fx, fy, fz);
+  if (rc) return rc;
+  if ((rc = bound(c, unew, vnew, wnew, 3, 0))) return rc;
+  k_fillps<<<nblocks(c->ncomp, 256), 256, 0, c->stream>>>(c->g, c->dt, unew, vnew, wnew, c->work[0]);
+  HIPF(c, hipGetLastError());
+  if ((rc = solve_compact(c, c->work[0], 1, p)) || (rc = bound(c, p, nullptr, nullptr, 1, 2)))
+    return rc;
+  k_correc<<<nblocks(c->ncomp, 256), 256, 0, c->stream>>>(c->g, c->dt, unew, vnew, wnew, p);
+  HIPF(c, hipGetLastError());
+  if ((rc = bound(c, unew, vnew, wnew, 3, 0))) return rc;
+  k_addp<<<nblocks(c->ncomp, 256), 256, 0, c->stream>>>(c->g, pnew, p);
+  HIPF(c, hipGetLastError());
+  return bound(c, pnew, nullptr, nullptr, 1, 2);
+}
+
+int fld_chkdiv_dev(fld_ctx* c, const double* u, const double* v, const double* w,
+                   double* divtot, double* divmax) {
+  if (!c) return FLD_E_ARG;
+  if (!(u && v && w && divtot && divmax)) return fail(c, FLD_E_ARG, "null pointer");
+  HIPF(c, hipSetDevice(c->device));
+  const unsigned nb = std::min<unsigned>(1024, nblocks(c->ncomp, 256));
+  k_div<<<nb, 256, 0, c->stream>>>(c->g, u, v, w, c->part);
+  HIPF(c, hipGetLastError());
+  std::vector<double> h(2 * (size_t)nb);
+  HIPF(c, hipMemcpyAsync(h.data(), c->part, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIPF(c, hipStreamSynchronize(c->stream));
+  double s = 0.0, m = 0.0;
+  for (unsigned b = 0; b < nb; ++b) s += h[2 * b], m = std::max(m, h[2 * b + 1]);
+  *divtot = s, *divmax = m;
+  return FLD_OK;
+}
+
+}  // extern "C"

@@ -112,3 +112,116 @@ def update_pos(grid, off, dt, x, y, z, ul, vl, wl, xold=None, yold=None):
     nc = lib().ora_update_pos(ctypes.byref(gs), ncap, _p(off), float(dt), _p(x), _p(y), _p(z),
                               _p(xold), _p(yold), _p(ul), _p(vl), _p(wl), _p(cen), _p(um))
     return cen, um, int(nc)
+
+
+# ---- the flow update (oracle/fluid_oracle.c) ---------------------------------------------
+def _flib():
+    o = lib()
+    if not getattr(o, "_fluid_sig", False):
+        G = ctypes.POINTER(IbxGrid)
+        D, I = ctypes.c_double, ctypes.c_int
+        o.ora_rfftf.argtypes = [I, P]
+        o.ora_rfftb.argtypes = [I, P]
+        o.ora_initsolver.argtypes = [G, D, D, I, P, P, P, P]
+        o.ora_solver1d.argtypes = [G, D, D, I, P]
+        o.ora_mom_rhs.argtypes = [G, D, D] + [P] * 13
+        o.ora_fftuvw.argtypes = [G, D, D] + [P] * 13
+        o.ora_bounduvw.argtypes = [G, P, P, P]
+        o.ora_boundp.argtypes = [G, P]
+        o.ora_fillps.argtypes = [G, D, P, P, P, P]
+        o.ora_correc.argtypes = [G, D, P, P, P, P]
+        o.ora_fluid_step.argtypes = [G, D, D] + [P] * 14
+        o.ora_chkdiv.argtypes = [G, P, P, P, ctypes.POINTER(D), ctypes.POINTER(D)]
+        o._fluid_sig = True
+    return o
+
+
+def rfftf(x):
+    x = np.array(x, dtype=np.float64)
+    _flib().ora_rfftf(x.size, _p(x))
+    return x
+
+
+def rfftb(x):
+    x = np.array(x, dtype=np.float64)
+    _flib().ora_rfftb(x.size, _p(x))
+    return x
+
+
+def ref_vrfft(x, inverse=False):
+    """The reference's own compiled VFFTPACK (one sequence), or None where not built."""
+    r = ref_lib()
+    if r is None:
+        return None
+    r.ref_vrfft.argtypes = [ctypes.c_int, P, ctypes.c_int]
+    x = np.array(x, dtype=np.float64)
+    r.ref_vrfft(x.size, _p(x), 1 if inverse else 0)
+    return x
+
+
+def initsolver(grid, rep, dt, e):
+    k, n = grid.ktot, grid.itot * grid.jtot
+    a, b, c, xy = np.zeros(k), np.zeros(k), np.zeros(k), np.zeros(n)
+    gs = grid_struct(grid)
+    _flib().ora_initsolver(ctypes.byref(gs), rep, dt, e, _p(a), _p(b), _p(c), _p(xy))
+    return a, b, c, xy.reshape(grid.jtot, grid.itot).T
+
+
+def solver1d(grid, rep, dt, ne, p):
+    """In place on the interior of the Fortran-ordered field p."""
+    gs = grid_struct(grid)
+    _flib().ora_solver1d(ctypes.byref(gs), rep, dt, ne, _p(p))
+
+
+def mom_rhs(grid, rep, dt, uo, vo, wo, uoo, voo, woo, p, fx, fy, fz):
+    out = [grid.empty_field() for _ in range(3)]
+    gs = grid_struct(grid)
+    _flib().ora_mom_rhs(ctypes.byref(gs), rep, dt, *[_p(a) for a in (uo, vo, wo, uoo, voo, woo, p, fx, fy, fz)],
+                        *[_p(a) for a in out])
+    return out
+
+
+def fftuvw(grid, rep, dt, uo, vo, wo, uoo, voo, woo, pnew, fx, fy, fz):
+    out = [grid.empty_field() for _ in range(3)]
+    gs = grid_struct(grid)
+    _flib().ora_fftuvw(ctypes.byref(gs), rep, dt, *[_p(a) for a in out],
+                       *[_p(a) for a in (uo, vo, wo, uoo, voo, woo, pnew, fx, fy, fz)])
+    return out
+
+
+def bounduvw(grid, u, v, w):
+    gs = grid_struct(grid)
+    _flib().ora_bounduvw(ctypes.byref(gs), _p(u), _p(v), _p(w))
+
+
+def boundp(grid, p):
+    gs = grid_struct(grid)
+    _flib().ora_boundp(ctypes.byref(gs), _p(p))
+
+
+def fillps(grid, dt, u, v, w):
+    p = grid.empty_field()
+    gs = grid_struct(grid)
+    _flib().ora_fillps(ctypes.byref(gs), dt, _p(u), _p(v), _p(w), _p(p))
+    return p
+
+
+def correc(grid, dt, u, v, w, p):
+    gs = grid_struct(grid)
+    _flib().ora_correc(ctypes.byref(gs), dt, _p(u), _p(v), _p(w), _p(p))
+
+
+def fluid_step(grid, rep, dt, unew, vnew, wnew, pnew, uo, vo, wo, uoo, voo, woo, fx, fy, fz):
+    """main.f90:388-410 in place on unew, vnew, wnew, pnew; returns the correction pressure."""
+    p = grid.empty_field()
+    gs = grid_struct(grid)
+    _flib().ora_fluid_step(ctypes.byref(gs), rep, dt, *[_p(a) for a in (unew, vnew, wnew, pnew, p, uo, vo, wo,
+                                                                         uoo, voo, woo, fx, fy, fz)])
+    return p
+
+
+def chkdiv(grid, u, v, w):
+    gs = grid_struct(grid)
+    t, m = ctypes.c_double(), ctypes.c_double()
+    _flib().ora_chkdiv(ctypes.byref(gs), _p(u), _p(v), _p(w), ctypes.byref(t), ctypes.byref(m))
+    return t.value, m.value

@@ -43,3 +43,24 @@ def test_create_rejects_bad_grid_without_device():
     h = ctypes.c_void_p()
     assert lib.ibx_create(ctypes.byref(h), ctypes.byref(g), 0) == _lib.IBX_E_ARG
     assert not h.value
+
+
+def test_fluid_library_exports_every_declared_symbol():
+    lib = _lib.load_fluid()
+    names = _lib.declared_fluid_symbols()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(lib, n), n
+        assert getattr(lib, n).argtypes is not None, n
+
+
+def test_fluid_create_rejects_bad_grid_without_device():
+    """itot and jtot must be powers of two (radix-2 transforms), ktot >= 2: refused on the
+    host before any device call."""
+    lib = _lib.load_fluid()
+    h = ctypes.c_void_p()
+    for itot, jtot, ktot in ((24, 16, 8), (16, 12, 8), (16, 16, 1), (2, 16, 8)):
+        g = _lib.IbxGrid(itot, jtot, ktot, 0, itot * 0.1, jtot * 0.1, ktot * 0.1, 10, 10, 10,
+                         0.1, 0.1, 0.1, 0, 0)
+        assert lib.fld_create(ctypes.byref(h), ctypes.byref(g), 0.0016, 1e-4, 0) == -1
+        assert not h.value

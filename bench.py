@@ -158,6 +158,50 @@ def capsule_step_leg(torch, steps=20, warmup=3):
             "spread_config": "objects-w8 (what the autotuner picks for this workload)"}
 
 
+def coupled_leg(torch, steps=100, warmup=2):
+    """The coupled step at the dense channel (1024 x 512 x 256 cells, 8000 unit-radius capsules
+    of degree 16 at 30 %, h = 1/10.6; suguru_amd.coupled_step): interpolation, FP64 membrane
+    law, spread, Update_Pos and the flow update (FFT momentum and pressure solves), Rep = 0.0016,
+    dt = 1e-4, gs = 6250, eb = 0.3 gs.  `warmup` eager steps with HIP events between the stages
+    give the stage times; then `steps` replays of the three captured hipGraphs from the initial
+    state give the step time, the centroids and the mean capsule velocities after `steps`."""
+    from suguru_amd import workload as W
+    from suguru_amd.coupled_step import CoupledStep
+
+    t0 = time.perf_counter()
+    wl = W.config_dense_full()
+    g, caps = wl.grid, wl.caps
+    st = CoupledStep(g, caps.cap_off, degree=16)
+    t_setup = time.perf_counter() - t0
+    try:
+        st.load(wl.u, wl.v, wl.w, caps.x, caps.y, caps.z)
+        ev = [[torch.cuda.Event(enable_timing=True) for _ in range(6)] for _ in range(warmup)]
+        for k in range(warmup):
+            st.step(ev[k])
+        st.sync()
+        names = ("interp", "membrane_law", "spread", "update_pos", "fluid")
+        stage = np.array([[ev[k][i].elapsed_time(ev[k][i + 1]) for i in range(5)]
+                          for k in range(warmup)]) * 1e3
+        st.load(wl.u, wl.v, wl.w, caps.x, caps.y, caps.z)
+        st.capture()
+        st.sync()
+        a = time.perf_counter()
+        for _ in range(steps):
+            st.run()
+        st.sync()
+        ms = 1e3 * (time.perf_counter() - a) / steps
+        cen = st.centroid.view(-1, 3).cpu().numpy()
+        um = st.umean.view(-1, 3).cpu().numpy()
+    finally:
+        st.close()
+    return {"workload": wl.name + " at h = 1/10.6 (unit-radius capsules)",
+            "grid_itot_jtot_ktot": [g.itot, g.jtot, g.ktot], "capsules": caps.ncap,
+            "points": caps.npts, "steps": steps, "graph_ms_per_step": ms,
+            "stages_us_eager": {n: float(stage[-1, i]) for i, n in enumerate(names)},
+            "centroid_mean_after": cen.mean(0).tolist(),
+            "umean_mean_after": um.mean(0).tolist(), "setup_s": round(t_setup, 1)}
+
+
 def copy_peak_gbs(torch, reps=10):
     """Practical HBM bandwidth: a 1 GiB device-to-device copy (read + write = 2 GiB per copy),
     timed with events on torch's stream."""
@@ -202,6 +246,12 @@ def main():
                     help="run warmup + steps only (for rocprofv3), no extras")
     ap.add_argument("--no-capsule-step", action="store_true",
                     help="skip the capsule-step leg (interp, membrane law, spread, Update_Pos)")
+    ap.add_argument("--no-coupled", action="store_true",
+                    help="skip the coupled-step leg (dense 8000-capsule channel with the flow update)")
+    ap.add_argument("--coupled-only", action="store_true",
+                    help="run the coupled-step leg alone and print it")
+    ap.add_argument("--coupled-steps", type=int, default=100,
+                    help="graph replays of the coupled-step leg (default 100)")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the learned-traction leg (training + step-time comparison)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -223,6 +273,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copied_bytes": a.numel() * 8, "copies": 3}))
+        return
+
+    if args.coupled_only:
+        print(json.dumps({"coupled_step": coupled_leg(torch, steps=args.coupled_steps)}))
         return
 
     from suguru_amd import Batch, IBTransfer, workload as W
@@ -382,6 +436,11 @@ def main():
             out["capsule_step"] = capsule_step_leg(torch)
         except Exception as e:  # reported, never hides the transfer line
             out["capsule_step"] = {"error": f"{type(e).__name__}: {e}"}
+    if not args.no_coupled:
+        try:
+            out["coupled_step"] = coupled_leg(torch, steps=args.coupled_steps)
+        except Exception as e:  # reported, never hides the transfer line
+            out["coupled_step"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

@@ -1,0 +1,190 @@
+"""The coupled immersed-boundary time step on one GPU, in the reference's order.
+
+Per step the reference runs (main.f90:188-410, fsolver = "f", no overlap mesh):
+
+    uoo <- uo <- unew                                     (main.f90:197-205)
+    eulr2lagr -> complagrforces -> (collisions) -> lagr2eulr -> (indicator) -> Update_Pos
+    -> (surf_vol) -> fftuvw -> bounduvw -> fillps -> solver1d(p) -> boundp -> correc
+    -> bounduvw -> pnew += p -> boundp(pnew)
+
+The dense configuration leaves out collisions, the indicator and surf_vol (diagnostics).
+Every stage is enqueued on one HIP stream, with no host round trip:
+
+    1. ibx_interp_f64_dev          velocity at every point             (eulr2lagr)
+    2. traction                    the FP64 law (memb_traction_f64_dev) or a surrogate
+    3. ibx_spread_f64_dev_noalloc  traction x area onto the grid       (lagr2eulr)
+    4. ibx_update_pos_dev          x += u dt, centroid, periodic shift (Update_Pos.f90:77-186)
+    5. fld_step_dev                the flow update                     (main.f90:388-410)
+
+The three velocity sets rotate by reference (no copies): the buffer that becomes unew is the
+old uoo, whose interior fftuvw rewrites and whose halos bounduvw rewrites.  The rotation has
+period three, so `capture()` records three hipGraphs, one per phase, and `run()` replays them
+in turn; the spread's no-allocation entry point keeps hipMalloc out of the captured work.
+
+The reference starts with uo = 0 (module arrays), so the first step's uoo is zero
+(main.f90:190-205); the same holds here.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .fluid import DT, REP, FlowSolver
+from .interp_spread import IBTransfer, _offsets
+from .membrane import MembranePlan
+
+GSCELL = 6250.0         # 1 / We, We = Rep * Ca = 0.0016 * 0.1 (param.f90:57,88)
+EBEND = 0.3 * GSCELL    # cbend * gscell (param.f90:91)
+
+
+class CoupledStep:
+    """Device state and the stage launches of the coupled step (module docstring).
+
+    grid: ibx-style grid (itot, jtot powers of two); cap_off: every capsule has the points of
+    degree `degree`, latitude-major; gs, eb: the law's moduli; rep, dt: param.f90:57 and
+    main.f90:119.  traction: None for the FP64 law, or an object with
+    ``enqueue(x, y, z, fx, fy, fz, nx, ny, nz, area)`` that writes the same outputs on the
+    step's stream without waiting (a surrogate).
+    """
+
+    def __init__(self, grid, cap_off, degree: int = 16, gs: float = GSCELL, eb: float = EBEND,
+                 rep: float = REP, dt: float = DT, device: int = 0, spread_config="objects-w8",
+                 traction=None):
+        import torch
+
+        self.torch = torch
+        off = _offsets(cap_off)
+        self.plan = MembranePlan(degree, device)
+        n = self.plan.npts
+        ncap = off.size - 1
+        if not np.array_equal(off, np.arange(ncap + 1, dtype=np.int64) * n):
+            self.plan.close()
+            raise ValueError(f"every capsule must have the {n} points of degree {degree}")
+        self.ctx = IBTransfer(grid, device)
+        if spread_config is not None:
+            self.ctx.set_config(spread_config)
+        self.fluid = FlowSolver(grid, rep, dt, device)
+        self.grid, self.ncap, self.npts = grid, ncap, ncap * n
+        self.gs, self.eb, self.dt = float(gs), float(eb), float(dt)
+        self.traction = traction
+        self.dev = torch.device("cuda", device)
+        self.off = torch.from_numpy(off).to(self.dev)
+        e = lambda k: torch.zeros(k, dtype=torch.float64, device=self.dev)
+        nf = grid.ncells_halo
+        # velocity sets [unew, uo, uoo] x (u, v, w); pressure, correction, spread force
+        self.vel = [[e(nf) for _ in range(3)] for _ in range(3)]
+        self.pnew, self.p = e(nf), e(nf)
+        self.force = [e(nf) for _ in range(3)]
+        self.x, self.y, self.z = e(self.npts), e(self.npts), e(self.npts)
+        self.ul, self.vl, self.wl = e(self.npts), e(self.npts), e(self.npts)
+        self.fx, self.fy, self.fz = e(self.npts), e(self.npts), e(self.npts)
+        self.nx, self.ny, self.nz = e(self.npts), e(self.npts), e(self.npts)
+        self.area = e(self.npts)
+        self.centroid, self.umean = e(3 * ncap), e(3 * ncap)
+        self.stream = torch.cuda.Stream(self.dev)
+        s = self.stream.cuda_stream
+        self.ctx.set_stream(s)
+        self.plan.set_stream(s)
+        self.fluid.set_stream(s)
+        self.phase = 0          # which velocity set is unew
+        self.graphs = None
+        self.steps_done = 0
+
+    # ---- state ------------------------------------------------------------------------------
+    def load(self, u, v, w, x, y, z, pnew=None):
+        """Initial state (numpy Fortran-ordered fields, point arrays): unew = (u, v, w) with
+        halos made consistent by bounduvw, uo = uoo = 0, pnew (default 0)."""
+        t = self.torch
+
+        def up(a):
+            return t.from_numpy(np.ascontiguousarray(np.asarray(a).reshape(-1, order="F"))).to(self.dev)
+
+        with t.cuda.stream(self.stream):
+            for k, a in enumerate((u, v, w)):
+                self.vel[0][k].copy_(up(a))
+            for st in (1, 2):
+                for k in range(3):
+                    self.vel[st][k].zero_()
+            self.pnew.copy_(up(pnew) if pnew is not None else t.zeros_like(self.pnew))
+            self.p.zero_()
+            for dst, a in zip((self.x, self.y, self.z), (x, y, z)):
+                dst.copy_(up(a))
+        self.phase = 0
+        self.fluid.bounduvw(*self.vel[0])
+        self.fluid.boundp(self.pnew)
+        self.sync()
+
+    def current(self):
+        """(u, v, w) of the current unew set."""
+        return self.vel[self.phase]
+
+    # ---- one step -----------------------------------------------------------------------------
+    def _sets(self, phase):
+        """unew, uo, uoo sets after the rotation that starts the step at `phase` (the index of
+        the set holding the current unew)."""
+        uo = self.vel[phase]            # uo <- unew
+        uoo = self.vel[(phase + 1) % 3]  # uoo <- uo (the previous step's uo)
+        unew = self.vel[(phase + 2) % 3]  # the freed old uoo
+        return unew, uo, uoo
+
+    def _enqueue(self, phase, events=None):
+        unew, uo, uoo = self._sets(phase)
+        mark = (lambda i: events[i].record(self.stream)) if events is not None else (lambda i: None)
+        u, v, w = uo  # the velocity of this step: unew before the rotation (main.f90:225)
+        mark(0)
+        self.ctx.interp_dev(u, v, w, self.off, self.x, self.y, self.z, self.ul, self.vl, self.wl)
+        mark(1)
+        if self.traction is None:
+            self.plan.traction_dev(self.x, self.y, self.z, self.gs, self.eb, self.fx, self.fy,
+                                   self.fz, self.nx, self.ny, self.nz, self.area)
+        else:
+            self.traction.enqueue(self.x, self.y, self.z, self.fx, self.fy, self.fz, self.nx,
+                                  self.ny, self.nz, self.area)
+        mark(2)
+        self.ctx.spread_dev(self.off, self.x, self.y, self.z, self.fx, self.fy, self.fz,
+                            self.area, *self.force, noalloc=True)
+        mark(3)
+        self.ctx.update_pos_dev(self.off, self.x, self.y, self.z, self.ul, self.vl, self.wl,
+                                self.dt, self.centroid, self.umean)
+        mark(4)
+        self.fluid.step(*unew, self.pnew, self.p, *uo, *uoo, *self.force)
+        mark(5)
+
+    def step(self, events=None):
+        """One eager step (asynchronous on the step's stream).  events: six torch.cuda.Event
+        recorded before, between and after the five stages."""
+        self._enqueue(self.phase, events)
+        self.phase = (self.phase + 2) % 3
+        self.steps_done += 1
+
+    def capture(self):
+        """Record the three phases of the step as hipGraphs (torch.cuda.CUDAGraph on the step's
+        stream).  The state is not advanced: capture launches nothing."""
+        t = self.torch
+        self.graphs = []
+        for ph in range(3):
+            g = t.cuda.CUDAGraph()
+            with t.cuda.graph(g, stream=self.stream):
+                self._enqueue(ph)
+            self.graphs.append(g)
+
+    def run(self):
+        """One step by replaying the captured graph of the current phase."""
+        if self.graphs is None:
+            raise RuntimeError("capture() first")
+        with self.torch.cuda.stream(self.stream):
+            self.graphs[self.phase].replay()
+        self.phase = (self.phase + 2) % 3
+        self.steps_done += 1
+
+    def sync(self):
+        """Waits for the stream and raises the first error any library flagged."""
+        self.stream.synchronize()
+        self.ctx.sync()
+        self.plan.sync()
+        self.fluid.sync()
+
+    def close(self):
+        self.graphs = None
+        self.plan.close()
+        self.ctx.close()
+        self.fluid.close()

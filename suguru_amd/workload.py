@@ -378,5 +378,22 @@ def config_dense_share(seed: int = 1234, with_fields: bool = True,
     return Workload("share-128x512x256-1000cap-30pct", g, caps, u, v, w, fx, fy, fz)
 
 
+def config_dense_full(seed: int = 1234, with_fields: bool = True, h: float = 1.0 / 10.6,
+                      counts=(40, 20, 10), grid_cells=(1024, 512, 256)) -> Workload:
+    """The dense coupled channel: 1024 x 512 x 256 (itot, jtot, ktot) cells with 8000 capsules
+    of radius 10.6h and degree 16 (578 points each, 4 624 000 in all) at 30 % volume fraction,
+    on a jittered 40 x 20 x 10 lattice.  With h = 1/10.6 the capsules have unit radius, the
+    membrane law's stress-free size (param.f90:135 radius = 1).  counts / grid_cells scale the
+    same arrangement down (one lattice cell per 25.6 x 25.6 x 24.5 h)."""
+    g = make_grid(*grid_cells, h)
+    r = 10.6 * g.dx
+    centers = lattice_capsules(g, r, counts, seed + 11)
+    caps = build_capsules(centers, np.full(len(centers), r), [16] * len(centers))
+    u, v, w = velocity_field(g, seed) if with_fields else (None, None, None)
+    fx, fy, fz = forces(caps.npts, seed + 1)
+    name = f"dense-{grid_cells[0]}x{grid_cells[1]}x{grid_cells[2]}-{len(centers)}cap-30pct"
+    return Workload(name, g, caps, u, v, w, fx, fy, fz)
+
+
 def volume_fraction(grid: Grid, caps: Capsules) -> float:
     return float(np.sum(4.0 / 3.0 * math.pi * caps.radii**3) / (grid.lx * grid.ly * grid.lz))

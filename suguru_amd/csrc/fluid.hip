@@ -32,7 +32,7 @@ namespace {
 
 constexpr int FFT_THREADS = 256;
 constexpr int TRI_MODES = 64;        // modes (lanes) per elimination workgroup
-constexpr int Y_LINES = 16;          // columns per y-transform workgroup
+constexpr int Y_LINES = 8;           // columns per y-transform workgroup
 constexpr size_t LDS_MAX = 160 * 1024;
 
 struct FGrid {
@@ -57,22 +57,48 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 // columns.
 
 // radix-2 decimation in time over nl lines of M complex values (input in bit-reversed order)
+// Line l starts at l * (M + 1) complex values in LDS: the pad of one keeps the lines of a
+// workgroup in different banks when neighbouring threads work on neighbouring lines.
+// Decimation in time over nl lines of M complex values (input in bit-reversed order): one
+// radix-2 stage when log2 M is odd, then radix-4 passes, each the two radix-2 stages of spans
+// h and 2h on four values held in registers (half the LDS round trips and barriers).
 template <bool INV>
-__device__ void fft_stages(double2* s, int nl, int M, int N, const double2* __restrict__ tw) {
-  const int half_m = M >> 1;
-  const int nb = nl * half_m;
-  for (int h = 1, lg = 0; h < M; h <<= 1, ++lg) {
+__device__ void fft_stages(double2* s, int nl, int M, int logm, int logn,
+                           const double2* __restrict__ tw) {
+  int h = 1, lg = 0;
+  if (logm & 1) {
     __syncthreads();
-    const int tstride = N / (2 * h);
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-      const int line = b / half_m, r = b - line * half_m;
-      const int pos = r & (h - 1), grp = r >> lg;
-      const int i0 = line * M + (grp << (lg + 1)) + pos, i1 = i0 + h;
-      double2 w = tw[pos * tstride];
-      if (INV) w.y = -w.y;
-      const double2 a = s[i0], t = cmul(w, s[i1]);
+    const int half_m = M >> 1;
+    for (int b = threadIdx.x; b < nl * half_m; b += blockDim.x) {
+      const int line = b >> (logm - 1), r = b & (half_m - 1);
+      const int i0 = line * (M + 1) + 2 * r, i1 = i0 + 1;
+      const double2 a = s[i0], t = s[i1];  // twiddle W^0 = 1
       s[i0] = make_double2(a.x + t.x, a.y + t.y);
       s[i1] = make_double2(a.x - t.x, a.y - t.y);
+    }
+    h = 2, lg = 1;
+  }
+  const int quarter = M >> 2;
+  for (; h < M; h <<= 2, lg += 2) {
+    __syncthreads();
+    const int ts1 = logn - 1 - lg, ts2 = logn - 2 - lg;  // N / (2h), N / (4h)
+    for (int b = threadIdx.x; b < nl * quarter; b += blockDim.x) {
+      const int line = b >> (logm - 2), r = b & (quarter - 1);
+      const int pos = r & (h - 1), grp = r >> lg;
+      const int base = line * (M + 1) + (grp << (lg + 2)) + pos;
+      double2 w1 = tw[pos << ts1], w2 = tw[pos << ts2], w3 = tw[(pos + h) << ts2];
+      if (INV) w1.y = -w1.y, w2.y = -w2.y, w3.y = -w3.y;
+      const double2 a0 = s[base], a1 = s[base + h], a2 = s[base + 2 * h], a3 = s[base + 3 * h];
+      const double2 t1 = cmul(w1, a1), t3 = cmul(w1, a3);
+      const double2 b0 = make_double2(a0.x + t1.x, a0.y + t1.y);
+      const double2 b1 = make_double2(a0.x - t1.x, a0.y - t1.y);
+      const double2 b2 = make_double2(a2.x + t3.x, a2.y + t3.y);
+      const double2 b3 = make_double2(a2.x - t3.x, a2.y - t3.y);
+      const double2 u2 = cmul(w2, b2), u3 = cmul(w3, b3);
+      s[base] = make_double2(b0.x + u2.x, b0.y + u2.y);
+      s[base + 2 * h] = make_double2(b0.x - u2.x, b0.y - u2.y);
+      s[base + h] = make_double2(b1.x + u3.x, b1.y + u3.y);
+      s[base + 3 * h] = make_double2(b1.x - u3.x, b1.y - u3.y);
     }
   }
   __syncthreads();
@@ -93,20 +119,20 @@ __global__ void __launch_bounds__(FFT_THREADS)
                long long es_dst, long long ls_dst, long long off_src,
                const double2* __restrict__ tw) {
   extern __shared__ double2 sm[];
-  const int M = N >> 1;
+  const int M = N >> 1, logn = logm + 1, lognl = 31 - __clz(nl);
   const int kp = blockIdx.x / groups_per_plane, gp = blockIdx.x - kp * groups_per_plane;
   const long long bs = off_src + kp * plane_src + gp * gstride_src;
   const long long bd = kp * plane_dst + gp * gstride_dst;
   double* smd = reinterpret_cast<double*>(sm);
   for (int q = threadIdx.x; q < nl * N; q += blockDim.x) {
-    const int l = XMAJOR ? q / N : q % nl, e = XMAJOR ? q - (q / N) * N : q / nl;
-    smd[2 * (l * M + brev(e >> 1, logm)) + (e & 1)] = src[bs + l * ls_src + e * es_src];
+    const int l = XMAJOR ? q >> logn : q & (nl - 1), e = XMAJOR ? q & (N - 1) : q >> lognl;
+    smd[2 * (l * (M + 1) + brev(e >> 1, logm)) + (e & 1)] = src[bs + l * ls_src + e * es_src];
   }
-  fft_stages<false>(sm, nl, M, N, tw);
+  fft_stages<false>(sm, nl, M, logm, logn, tw);
   // X_k = E_k + W^k O_k, E_k = (Z_k + conj Z_M-k) / 2, O_k = (Z_k - conj Z_M-k) / 2i
   for (int q = threadIdx.x; q < nl * M; q += blockDim.x) {
-    const int l = XMAJOR ? q / M : q % nl, k = XMAJOR ? q - (q / M) * M : q / nl;
-    const double2* z = sm + l * M;
+    const int l = XMAJOR ? q >> logm : q & (nl - 1), k = XMAJOR ? q & (M - 1) : q >> lognl;
+    const double2* z = sm + l * (M + 1);
     double* out = dst + bd + l * ls_dst;
     if (k == 0) {
       const double2 z0 = z[0];
@@ -132,13 +158,13 @@ __global__ void __launch_bounds__(FFT_THREADS)
                long long es_dst, long long ls_dst, long long off_dst,
                const double2* __restrict__ tw) {
   extern __shared__ double2 sm[];
-  const int M = N >> 1;
+  const int M = N >> 1, logn = logm + 1, lognl = 31 - __clz(nl);
   const int kp = blockIdx.x / groups_per_plane, gp = blockIdx.x - kp * groups_per_plane;
   const long long bs = kp * plane_src + gp * gstride_src;
   const long long bd = off_dst + kp * plane_dst + gp * gstride_dst;
   // Z_k = E_k + i O_k, E_k = (X_k + conj X_M-k) / 2, O_k = (X_k - conj X_M-k) conj(W^k) / 2
   for (int q = threadIdx.x; q < nl * M; q += blockDim.x) {
-    const int l = XMAJOR ? q / M : q % nl, k = XMAJOR ? q - (q / M) * M : q / nl;
+    const int l = XMAJOR ? q >> logm : q & (nl - 1), k = XMAJOR ? q & (M - 1) : q >> lognl;
     const double* in = src + bs + l * ls_src;
     double2 Z;
     if (k == 0) {
@@ -153,14 +179,14 @@ __global__ void __launch_bounds__(FFT_THREADS)
       const double2 O = cmul(D, wc);
       Z = make_double2(E.x - O.y, E.y + O.x);
     }
-    sm[l * M + brev(k, logm)] = Z;
+    sm[l * (M + 1) + brev(k, logm)] = Z;
   }
-  fft_stages<true>(sm, nl, M, N, tw);
+  fft_stages<true>(sm, nl, M, logm, logn, tw);
   const double sc = 1.0 / (double)M;
   const double* smd = reinterpret_cast<const double*>(sm);
   for (int q = threadIdx.x; q < nl * N; q += blockDim.x) {
-    const int l = XMAJOR ? q / N : q % nl, e = XMAJOR ? q - (q / N) * N : q / nl;
-    dst[bd + l * ls_dst + e * es_dst] = smd[2 * (l * M + (e >> 1)) + (e & 1)] * sc;
+    const int l = XMAJOR ? q >> logn : q & (nl - 1), e = XMAJOR ? q & (N - 1) : q >> lognl;
+    dst[bd + l * ls_dst + e * es_dst] = smd[2 * (l * (M + 1) + (e >> 1)) + (e & 1)] * sc;
   }
 }
 
@@ -297,6 +323,15 @@ __device__ __forceinline__ double momzad(const FGrid& g, const double* Uo, const
          g.dzi * ((1.5 * (-wwkp + wwkm)) - (.5 * (-wwkpo + wwkmo)));
 }
 
+// Workgroup b runs on XCD b % 8 (round-robin dispatch).  The stencil kernels remap it so that
+// each XCD takes one contiguous eighth of the cells: the neighbouring rows and planes a cell
+// reads were then mostly loaded by the same XCD and hit its L2.  The grid is padded to a
+// multiple of 8 workgroups; the padding workgroups return.
+__device__ __forceinline__ long long xcd_block() {
+  const unsigned per = gridDim.x >> 3;
+  return (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+}
+
 // fftuvw.f90:30-105: the three right-hand sides of one interior cell, scaled by Rep, written
 // to the compact arrays (dwdt(:,:,kmax) = 0)
 __global__ void __launch_bounds__(256)
@@ -308,7 +343,7 @@ __global__ void __launch_bounds__(256)
               const double* __restrict__ fz, double* __restrict__ cu, double* __restrict__ cv,
               double* __restrict__ cw) {
   const long long n = (long long)g.itot * g.jtot * g.ktot;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int i = (int)(t % g.itot) + 1;
   const long long r = t / g.itot;
@@ -578,7 +613,7 @@ int fft_x(fld_ctx* c, bool inv, const double* src, bool src_field, double* dst, 
   const long long row_s = src_field ? g.sy : g.itot, plane_s = src_field ? g.sz : cp;
   const long long row_d = dst_field ? g.sy : g.itot, plane_d = dst_field ? g.sz : cp;
   const long long off = 1 + g.sy + g.sz;  // (1, 1, 1)
-  const size_t lds = (size_t)nl * N * sizeof(double);
+  const size_t lds = (size_t)nl * (N / 2 + 1) * sizeof(double2);
   const unsigned nb = (unsigned)(gpp * g.ktot);
   if (!inv)
     k_rfft_fwd<true><<<nb, FFT_THREADS, lds, c->stream>>>(
@@ -650,7 +685,8 @@ int fld_create(fld_ctx** out, const ibx_grid* gr, double rep, double dt, int dev
   if (!pow2(gr->itot) || !pow2(gr->jtot) || gr->itot < 4 || gr->jtot < 4 || gr->ktot < 2)
     return FLD_E_ARG;
   // one line of each transform must fit in LDS
-  if ((size_t)gr->itot * sizeof(double) > LDS_MAX || (size_t)gr->jtot * sizeof(double) > LDS_MAX)
+  if ((size_t)(gr->itot / 2 + 1) * sizeof(double2) > LDS_MAX ||
+      (size_t)(gr->jtot / 2 + 1) * sizeof(double2) > LDS_MAX)
     return FLD_E_ARG;
   if (!(rep > 0 && dt > 0 && gr->dxi > 0 && gr->dyi > 0 && gr->dzi > 0)) return FLD_E_ARG;
   fld_ctx* c = new fld_ctx();
@@ -665,9 +701,13 @@ int fld_create(fld_ctx** out, const ibx_grid* gr, double rep, double dt, int dev
   c->logmx = ilog2(g.itot / 2), c->logmy = ilog2(g.jtot / 2);
   c->xlines = std::min(g.jtot, std::max(1, 2048 / g.itot));
   c->ylines = std::min(g.itot, Y_LINES);
-  while (c->ylines > 1 && (size_t)c->ylines * g.jtot * sizeof(double) > LDS_MAX) c->ylines >>= 1;
-  c->lds_y = (size_t)c->ylines * g.jtot * sizeof(double);
-  c->tri_lds = (size_t)TRI_MODES * g.ktot * sizeof(double) <= LDS_MAX;
+  while (c->ylines > 1 && (size_t)c->ylines * (g.jtot / 2 + 1) * sizeof(double2) > LDS_MAX / 2)
+    c->ylines >>= 1;
+  c->lds_y = (size_t)c->ylines * (g.jtot / 2 + 1) * sizeof(double2);
+  // the sweep's forward values in LDS limit a compute unit to one 64-lane workgroup at
+  // ktot = 256; through the spectral array itself every compute unit runs many wavefronts
+  // and the extra pass is cheaper than the idle lanes
+  c->tri_lds = false;
   auto bail = [&](int code, const char* what) {
     fld_destroy(c);
     (void)what;
@@ -785,7 +825,7 @@ int fld_fftuvw_dev(fld_ctx* c, double* unew, double* vnew, double* wnew, const d
   if (!(unew && vnew && wnew && uo && vo && wo && uoo && voo && woo && pnew && fx && fy && fz))
     return fail(c, FLD_E_ARG, "null field");
   HIPF(c, hipSetDevice(c->device));
-  k_mom_rhs<<<nblocks(c->ncomp, 256), 256, 0, c->stream>>>(c->g, c->rep, c->dt, uo, vo, wo, uoo,
+  k_mom_rhs<<<(nblocks(c->ncomp, 256) + 7) & ~7u, 256, 0, c->stream>>>(c->g, c->rep, c->dt, uo, vo, wo, uoo,
                                                            voo, woo, pnew, fx, fy, fz, c->work[0],
                                                            c->work[1], c->work[2]);
   HIPF(c, hipGetLastError());

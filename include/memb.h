@@ -75,6 +75,30 @@ int memb_coeffs_f64_dev(memb_plan* p, int64_t ncap, const double* x, const doubl
 int memb_geometry_f64_dev(memb_plan* p, int64_t ncap, const double* x, const double* y,
                           const double* z, double* nx, double* ny, double* nz, double* area);
 
+/* The FP64 law for the capsules c with sel[c] != 0 only (device int array of ncap); the
+ * outputs of the other capsules are left as they are.  The surrogate's fallback: capsules
+ * its gate rejects are recomputed in place, with no host round trip (graph-capturable). */
+int memb_traction_select_f64_dev(memb_plan* p, int64_t ncap, const int* sel, double gs,
+                                 double eb, const double* x, const double* y, const double* z,
+                                 double* fx, double* fy, double* fz, double* nx, double* ny,
+                                 double* nz, double* area);
+
+/* ---- co-rotated frame of the surrogate (no reference counterpart) --------------------------- */
+/* The law is translation invariant and rotation covariant, f(R x + c) = R f(x), so the
+ * surrogate works on the deformation only.  Per capsule (X: the 3 x npts stress-free points,
+ * x block, y block, z block): centroid c, R = orthogonal polar factor of
+ * sum_p (x_p - c) X_p^T, delta_p = R^T (x_p - c) - X_p written as delta[c][3][npts], its RMS
+ * amplitude amp[c] (= +inf when the polar factor fails: det <= 0.5), the direction
+ * delta / amp in dhx, dhy, dhz (capsule-major, like x), and R row-major in rot[c][9]. */
+int memb_corot_f64_dev(memb_plan* p, int64_t ncap, const double* X, const double* x,
+                       const double* y, const double* z, double* delta, double* dhx,
+                       double* dhy, double* dhz, double* rot, double* amp);
+/* f = R (flin + amp^2 resid) per point: flin[c][3][npts] FP64 (the linear response in the
+ * co-rotated frame), resid[c][3][npts] FP32 (the learned residual per amp^2; NULL = 0). */
+int memb_corot_apply_f64_dev(memb_plan* p, int64_t ncap, const double* flin, const float* resid,
+                             const double* rot, const double* amp, double* fx, double* fy,
+                             double* fz);
+
 /* ---- learned surrogate: MLP, BF16 MFMA with FP32 accumulation ----------------------------- */
 /* Layers: din -> w (SiLU) -> w (SiLU) -> ... (nhidden layers of width w) -> dout, linear
  * output.  Weights are FP32 masters; the GEMMs read BF16 copies, refreshed whenever the

@@ -173,6 +173,9 @@ def load_memb() -> ctypes.CDLL:
         "memb_traction_f64_dev": (I, [P, I64, D, D] + [P] * 10),
         "memb_coeffs_f64_dev": (I, [P, I64, P, P, P, P]),
         "memb_geometry_f64_dev": (I, [P, I64, P, P, P, P, P, P, P]),
+        "memb_traction_select_f64_dev": (I, [P, I64, P, D, D] + [P] * 10),
+        "memb_corot_f64_dev": (I, [P, I64] + [P] * 10),
+        "memb_corot_apply_f64_dev": (I, [P, I64] + [P] * 7),
         "memb_mlp_create": (I, [ctypes.POINTER(P), I, I, I, I, I, I]),
         "memb_mlp_destroy": (None, [P]),
         "memb_mlp_last_error": (ctypes.c_char_p, [P]),

@@ -385,7 +385,7 @@ __global__ void __launch_bounds__(THREADS, 2)
           const double* __restrict__ y, const double* __restrict__ z, double* __restrict__ o0,
           double* __restrict__ o1, double* __restrict__ o2, double* __restrict__ o3,
           double* __restrict__ o4, double* __restrict__ o5, double* __restrict__ o6,
-          double* __restrict__ scratch) {
+          double* __restrict__ scratch, const int* __restrict__ sel) {
   extern __shared__ double lds[];
   const int tid = threadIdx.x;
   const int N = T.N, NN = N * N, np2 = T.np2, nt = T.nt, np = T.np;
@@ -395,6 +395,7 @@ __global__ void __launch_bounds__(THREADS, 2)
   double* U = cb + 3 * NN;     // work region
   double* scr = scratch + (size_t)blockIdx.x * nt * 6 * N * 2;  // >= 6 nc doubles too
   for (long long cap = blockIdx.x; cap < ncap; cap += gridDim.x) {
+    if (sel && sel[cap] == 0) continue;  // (uniform in the workgroup)
     const double* xyz[3] = {x + cap * nc, y + cap * nc, z + cap * nc};
     // 1. analysis: longitude DFT per (c, i, m) into U, then quadrature per (c, m, n)
     for (int t = tid; t < 3 * NN; t += THREADS) {
@@ -743,6 +744,135 @@ int upload(memb_plan* p, const std::vector<double>& h, const double** dst) {
   return MEMB_OK;
 }
 
+// ---- co-rotated frame of the surrogate ---------------------------------------------------------
+// The law is invariant under translation and covariant under rotation of the current shape
+// (f(R x + c) = R f(x)): its load depends on the capsule's deformation only.  One workgroup
+// per capsule: centroid c, the cross-covariance H = sum (x_p - c) X_p^T against the
+// stress-free points X, the rotation R = the orthogonal polar factor of H (the rotation that
+// brings X closest to the shape, Kabsch), the displacement delta_p = R^T (x_p - c) - X_p, its
+// RMS amplitude s and the direction delta / s.
+constexpr int CR_THREADS = 256;
+
+__device__ inline void inv3t(const double A[3][3], double B[3][3], double& det) {
+  // B = inverse(A)^T
+  const double c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+  const double c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+  const double c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+  det = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+  const double id = 1.0 / det;
+  B[0][0] = c00 * id, B[0][1] = c01 * id, B[0][2] = c02 * id;
+  B[1][0] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * id;
+  B[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * id;
+  B[1][2] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) * id;
+  B[2][0] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * id;
+  B[2][1] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * id;
+  B[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * id;
+}
+
+__global__ void __launch_bounds__(CR_THREADS)
+    k_corot(int n, const double* __restrict__ X, const double* __restrict__ x,
+            const double* __restrict__ y, const double* __restrict__ z,
+            double* __restrict__ delta, double* __restrict__ dhx, double* __restrict__ dhy,
+            double* __restrict__ dhz, double* __restrict__ rot, double* __restrict__ amp) {
+  __shared__ double red[12][CR_THREADS / 64];
+  __shared__ double sh[12];
+  const long long cap = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const double* px = x + cap * n;
+  const double* py = y + cap * n;
+  const double* pz = z + cap * n;
+  auto block_sum = [&](double* v, int k) {  // sums v[0..k) over the workgroup into sh
+    for (int a = 0; a < k; ++a) {
+      double t = v[a];
+      for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o);
+      if (lane == 0) red[a][wv] = t;
+    }
+    __syncthreads();
+    if (tid < k) {
+      double t = 0.0;
+      for (int w = 0; w < CR_THREADS / 64; ++w) t += red[tid][w];
+      sh[tid] = t;
+    }
+    __syncthreads();
+  };
+  double v[12];
+  v[0] = v[1] = v[2] = 0.0;
+  for (int p = tid; p < n; p += CR_THREADS) v[0] += px[p], v[1] += py[p], v[2] += pz[p];
+  block_sum(v, 3);
+  const double c[3] = {sh[0] / n, sh[1] / n, sh[2] / n};
+  __syncthreads();
+  for (int a = 0; a < 9; ++a) v[a] = 0.0;
+  for (int p = tid; p < n; p += CR_THREADS) {
+    const double d[3] = {px[p] - c[0], py[p] - c[1], pz[p] - c[2]};
+    const double r[3] = {X[p], X[n + p], X[2 * n + p]};
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) v[3 * a + b] += d[a] * r[b];
+  }
+  block_sum(v, 9);
+  __shared__ double R[3][3];
+  __shared__ int ok;
+  if (tid == 0) {
+    // polar factor by Newton's iteration Y <- (Y + Y^-T) / 2 from the scaled H
+    double Y[3][3], Z[3][3], det = 0.0, nrm = 0.0;
+    for (int a = 0; a < 9; ++a) nrm += sh[a] * sh[a];
+    nrm = sqrt(nrm / 3.0);
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) Y[a][b] = nrm > 0.0 ? sh[3 * a + b] / nrm : (a == b ? 1.0 : 0.0);
+    for (int it = 0; it < 24; ++it) {
+      inv3t(Y, Z, det);
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) Y[a][b] = 0.5 * (Y[a][b] + Z[a][b]);
+    }
+    inv3t(Y, Z, det);
+    ok = isfinite(det) && det > 0.5;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) R[a][b] = Y[a][b], rot[cap * 9 + 3 * a + b] = Y[a][b];
+  }
+  __syncthreads();
+  // delta = R^T (x - c) - X, layout [cap][3][n]
+  double ss = 0.0;
+  for (int p = tid; p < n; p += CR_THREADS) {
+    const double d[3] = {px[p] - c[0], py[p] - c[1], pz[p] - c[2]};
+    for (int a = 0; a < 3; ++a) {
+      const double q = R[0][a] * d[0] + R[1][a] * d[1] + R[2][a] * d[2] - X[a * n + p];
+      delta[(cap * 3 + a) * n + p] = q;
+      ss += q * q;
+    }
+  }
+  v[0] = ss;
+  block_sum(v, 1);
+  const double s = sqrt(sh[0] / n);
+  const double inv = s > 0.0 ? 1.0 / s : 0.0;
+  for (int p = tid; p < n; p += CR_THREADS) {
+    dhx[cap * n + p] = delta[(cap * 3 + 0) * n + p] * inv;
+    dhy[cap * n + p] = delta[(cap * 3 + 1) * n + p] * inv;
+    dhz[cap * n + p] = delta[(cap * 3 + 2) * n + p] * inv;
+  }
+  if (tid == 0) amp[cap] = ok ? s : __builtin_inf();
+}
+
+// f = R (flin + s^2 resid) per point, into the capsule-major traction arrays
+__global__ void __launch_bounds__(256)
+    k_corot_apply(long long ncap, int n, const double* __restrict__ flin,
+                  const float* __restrict__ resid, const double* __restrict__ rot,
+                  const double* __restrict__ amp, double* __restrict__ fx,
+                  double* __restrict__ fy, double* __restrict__ fz) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncap * n) return;
+  const long long cap = q / n;
+  const int p = (int)(q - cap * n);
+  const double s2 = amp[cap] * amp[cap];
+  double g[3];
+  for (int a = 0; a < 3; ++a) {
+    const long long o = (cap * 3 + a) * n + p;
+    g[a] = flin[o] + (resid ? s2 * (double)resid[o] : 0.0);
+  }
+  const double* R = rot + cap * 9;
+  fx[q] = R[0] * g[0] + R[1] * g[1] + R[2] * g[2];
+  fy[q] = R[3] * g[0] + R[4] * g[1] + R[5] * g[2];
+  fz[q] = R[6] * g[0] + R[7] * g[1] + R[8] * g[2];
+}
+
 size_t lds_bytes(const Tables& T) {
   const int N = T.N, NN = N * N;
   size_t work = std::max<size_t>(3 * NN * 2, (size_t)RING * 3 * 5 * N * 2 + (size_t)RING * T.np * 6);
@@ -753,11 +883,11 @@ size_t lds_bytes(const Tables& T) {
 template <int MODE>
 int launch(memb_plan* p, long long ncap, double gs, double eb, const double* x, const double* y,
            const double* z, double* o0, double* o1, double* o2, double* o3, double* o4,
-           double* o5, double* o6) {
+           double* o5, double* o6, const int* sel = nullptr) {
   const int nwg = (int)std::min<long long>(ncap, p->num_wg);
   if (nwg <= 0) return MEMB_OK;
   k_mem<MODE><<<nwg, THREADS, p->lds, p->stream>>>(p->T, ncap, gs, eb, x, y, z, o0, o1, o2, o3,
-                                                   o4, o5, o6, p->scratch);
+                                                   o4, o5, o6, p->scratch, sel);
   MCHK(p, hipGetLastError());
   return MEMB_OK;
 }
@@ -921,6 +1051,51 @@ int memb_geometry_f64_dev(memb_plan* p, int64_t ncap, const double* x, const dou
     return fail(p, MEMB_E_ARG, "null pointer");
   MCHK(p, hipSetDevice(p->device));
   return launch<3>(p, ncap, 0.0, 0.0, x, y, z, nullptr, nullptr, nullptr, nx, ny, nz, area);
+}
+
+int memb_traction_select_f64_dev(memb_plan* p, int64_t ncap, const int* sel, double gs,
+                                 double eb, const double* x, const double* y, const double* z,
+                                 double* fx, double* fy, double* fz, double* nx, double* ny,
+                                 double* nz, double* area) {
+  if (!p) return MEMB_E_ARG;
+  if (!p->T.ref) return fail(p, MEMB_E_STATE, "plan not initialised");
+  if (ncap < 0) return fail(p, MEMB_E_ARG, "ncap < 0");
+  if (ncap > 0 && !(sel && x && y && z && fx && fy && fz && nx && ny && nz && area))
+    return fail(p, MEMB_E_ARG, "null pointer");
+  MCHK(p, hipSetDevice(p->device));
+  return launch<0>(p, ncap, gs, eb, x, y, z, fx, fy, fz, nx, ny, nz, area, sel);
+}
+
+int memb_corot_f64_dev(memb_plan* p, int64_t ncap, const double* X, const double* x,
+                       const double* y, const double* z, double* delta, double* dhx,
+                       double* dhy, double* dhz, double* rot, double* amp) {
+  if (!p) return MEMB_E_ARG;
+  if (ncap < 0) return fail(p, MEMB_E_ARG, "ncap < 0");
+  if (ncap > 0 && !(X && x && y && z && delta && dhx && dhy && dhz && rot && amp))
+    return fail(p, MEMB_E_ARG, "null pointer");
+  if (ncap == 0) return MEMB_OK;
+  MCHK(p, hipSetDevice(p->device));
+  k_corot<<<(unsigned)ncap, CR_THREADS, 0, p->stream>>>(p->T.N * p->T.np2, X, x, y, z, delta,
+                                                         dhx, dhy, dhz, rot, amp);
+  MCHK(p, hipGetLastError());
+  return MEMB_OK;
+}
+
+int memb_corot_apply_f64_dev(memb_plan* p, int64_t ncap, const double* flin, const float* resid,
+                             const double* rot, const double* amp, double* fx, double* fy,
+                             double* fz) {
+  if (!p) return MEMB_E_ARG;
+  if (ncap < 0) return fail(p, MEMB_E_ARG, "ncap < 0");
+  if (ncap > 0 && !(flin && rot && amp && fx && fy && fz))
+    return fail(p, MEMB_E_ARG, "null pointer");
+  if (ncap == 0) return MEMB_OK;
+  MCHK(p, hipSetDevice(p->device));
+  const int n = p->T.N * p->T.np2;
+  const long long tot = ncap * n;
+  k_corot_apply<<<(unsigned)((tot + 255) / 256), 256, 0, p->stream>>>(ncap, n, flin, resid, rot,
+                                                                      amp, fx, fy, fz);
+  MCHK(p, hipGetLastError());
+  return MEMB_OK;
 }
 
 }  // extern "C"

@@ -134,6 +134,93 @@ __global__ void __launch_bounds__(THREADS)
       }
 }
 
+// Forward GEMM of the larger batches: C[M x N] = A[M x K] W^T (A row-major, W [N x K]
+// row-major), 128 x 128 tile per 256-thread workgroup, each wavefront a 64 x 64 quarter as
+// 4 x 4 MFMA 16x16x32 blocks.  Two LDS buffers: while the MFMAs consume K-slice s, the
+// slice s + 1 is already in flight from HBM into registers and is stored to the other
+// buffer afterwards, so one barrier per slice separates the two.  Each output element adds
+// its K-slices in the same order as k_gemm, so both kernels give the same bits.
+template <int EPI>
+__global__ void __launch_bounds__(THREADS)
+    k_gemm_fwd(int M, int N, int K, const __bf16* __restrict__ Ag, int lda,
+               const __bf16* __restrict__ Bg, int ldb, const float* __restrict__ bias,
+               float* __restrict__ out0, __bf16* __restrict__ out1, int ldc) {
+  constexpr int BM = 128, BN = 128, F = 4;
+  __shared__ __attribute__((aligned(16))) __bf16 As[2][BM * LDK];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][BN * LDK];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
+  // staging: thread t moves rows t >> 2 and 64 + (t >> 2), 8 k-values at (t & 3) * 8
+  const int sr = tid >> 2, sk = (tid & 3) * 8;
+  const __bf16* ga0 = Ag + (size_t)(m0 + sr) * lda + sk;
+  const __bf16* ga1 = Ag + (size_t)(m0 + 64 + sr) * lda + sk;
+  const __bf16* gb0 = Bg + (size_t)(n0 + sr) * ldb + sk;
+  const __bf16* gb1 = Bg + (size_t)(n0 + 64 + sr) * ldb + sk;
+  bf16x8 ra0 = *reinterpret_cast<const bf16x8*>(ga0), ra1 = *reinterpret_cast<const bf16x8*>(ga1);
+  bf16x8 rb0 = *reinterpret_cast<const bf16x8*>(gb0), rb1 = *reinterpret_cast<const bf16x8*>(gb1);
+  f32x4 acc[F][F];
+#pragma unroll
+  for (int i = 0; i < F; ++i)
+#pragma unroll
+    for (int j = 0; j < F; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  *reinterpret_cast<bf16x8*>(&As[0][sr * LDK + sk]) = ra0;
+  *reinterpret_cast<bf16x8*>(&As[0][(64 + sr) * LDK + sk]) = ra1;
+  *reinterpret_cast<bf16x8*>(&Bs[0][sr * LDK + sk]) = rb0;
+  *reinterpret_cast<bf16x8*>(&Bs[0][(64 + sr) * LDK + sk]) = rb1;
+  __syncthreads();
+  int cur = 0;
+  for (int k0 = 0; k0 < K; k0 += TK) {
+    const bool more = k0 + TK < K;
+    if (more) {
+      ra0 = *reinterpret_cast<const bf16x8*>(ga0 + k0 + TK);
+      ra1 = *reinterpret_cast<const bf16x8*>(ga1 + k0 + TK);
+      rb0 = *reinterpret_cast<const bf16x8*>(gb0 + k0 + TK);
+      rb1 = *reinterpret_cast<const bf16x8*>(gb1 + k0 + TK);
+    }
+    bf16x8 a[F], b[F];
+#pragma unroll
+    for (int i = 0; i < F; ++i)
+      a[i] = *reinterpret_cast<const bf16x8*>(&As[cur][(wm + i * 16 + (lane & 15)) * LDK + 8 * (lane >> 4)]);
+#pragma unroll
+    for (int j = 0; j < F; ++j)
+      b[j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][(wn + j * 16 + (lane & 15)) * LDK + 8 * (lane >> 4)]);
+#pragma unroll
+    for (int i = 0; i < F; ++i)
+#pragma unroll
+      for (int j = 0; j < F; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    if (more) {
+      const int nx = cur ^ 1;
+      *reinterpret_cast<bf16x8*>(&As[nx][sr * LDK + sk]) = ra0;
+      *reinterpret_cast<bf16x8*>(&As[nx][(64 + sr) * LDK + sk]) = ra1;
+      *reinterpret_cast<bf16x8*>(&Bs[nx][sr * LDK + sk]) = rb0;
+      *reinterpret_cast<bf16x8*>(&Bs[nx][(64 + sr) * LDK + sk]) = rb1;
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int i = 0; i < F; ++i)
+#pragma unroll
+    for (int j = 0; j < F; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+        const int col = n0 + wn + j * 16 + (lane & 15);
+        const size_t o = (size_t)row * ldc + col;
+        const float v = acc[i][j][r];
+        if (EPI == EPI_BIAS_SILU) {
+          const float zz = v + bias[col];
+          out0[o] = zz;
+          out1[o] = to_bf16(silu(zz));
+        } else {
+          out0[o] = v + bias[col];
+        }
+      }
+  (void)M, (void)N;
+}
+
 // x[batch][din] fp32 -> X[Mp][Dp] bf16 with (x - mean) * scale, zero padding
 __global__ void k_input(int batch, int din, int Mp, int Dp, const float* __restrict__ x,
                         const float* __restrict__ mean, const float* __restrict__ scale,
@@ -224,6 +311,7 @@ __global__ void k_to_bf16(long long n, const float* __restrict__ a, __bf16* __re
 }
 
 inline int pad64(int v) { return (v + 63) / 64 * 64; }
+inline int pad128(int v) { return (v + 127) / 128 * 128; }  // batch rows
 inline unsigned nblk(long long n, int t = 256) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace mlp
@@ -282,6 +370,15 @@ template <bool TA, bool TB, int EPI>
 int gemm(memb_mlp* m, int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb,
          const float* bias, const float* aux, float* out0, void* out1, int ldc) {
   if (M % 64 || N % 64 || K % 32) return mfail(m, MEMB_E_ARG, "gemm shape not padded");
+  if constexpr (!TA && TB && (EPI == EPI_BIAS_SILU || EPI == EPI_BIAS)) {
+    // forward layers of batches large enough to fill the chip with 128 x 128 tiles
+    if (M % 128 == 0 && N % 128 == 0 && (long long)(M / 128) * (N / 128) >= m->cus) {
+      k_gemm_fwd<EPI><<<dim3(N / 128, M / 128), THREADS, 0, m->stream>>>(
+          M, N, K, A, lda, B, ldb, bias, out0, reinterpret_cast<__bf16*>(out1), ldc);
+      XCHK(m, hipGetLastError());
+      return MEMB_OK;
+    }
+  }
   if ((long long)(M / TM) * (N / TN) < 2LL * m->cus) {
     // too few 64 x 64 tiles to fill the compute units (training batches): 32 x 32 tiles
     k_gemm<32, 32, TA, TB, EPI><<<dim3(N / 32, M / 32), THREADS, 0, m->stream>>>(
@@ -295,7 +392,7 @@ int gemm(memb_mlp* m, int M, int N, int K, const __bf16* A, int lda, const __bf1
 }
 
 int forward(memb_mlp* m, int batch, const float* x) {
-  const int Mp = pad64(batch);
+  const int Mp = pad128(batch);
   k_input<<<nblk((long long)Mp * m->Dp), 256, 0, m->stream>>>(batch, m->din, Mp, m->Dp, x, m->mean,
                                                             m->scale, m->X);
   const __bf16* in = m->X;
@@ -318,7 +415,7 @@ int forward(memb_mlp* m, int batch, const float* x) {
 int backward(memb_mlp* m, int batch, const float* x, const float* t, const float* wt, float* loss) {
   int rc = forward(m, batch, x);
   if (rc) return rc;
-  const int Mp = pad64(batch);
+  const int Mp = pad128(batch);
   k_zero<<<1, 64, 0, m->stream>>>(m->loss, 1);
   k_loss<<<nblk((long long)Mp * m->Op), 256, 0, m->stream>>>(batch, m->dout, Mp, m->Op,
                                                             m->omean, m->oscale, m->Y, t, wt,
@@ -366,7 +463,7 @@ int memb_mlp_create(memb_mlp** out, int din, int width, int nhidden, int dout, i
   m->stream = m->own;
   (void)hipDeviceGetAttribute(&m->cus, hipDeviceAttributeMultiprocessorCount, device);
   m->din = din, m->width = width, m->nh = nhidden, m->dout = dout, m->maxb = max_batch;
-  m->Dp = pad64(din), m->Wp = pad64(width), m->Op = pad64(dout), m->Mp = pad64(max_batch);
+  m->Dp = pad64(din), m->Wp = pad64(width), m->Op = pad64(dout), m->Mp = pad128(max_batch);
   m->nl = nhidden + 1;
   for (int i = 0; i < m->nl; ++i) {
     m->in_p.push_back(i == 0 ? m->Dp : m->Wp);

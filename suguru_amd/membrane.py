@@ -66,6 +66,25 @@ class MembranePlan:
         self._check(self.lib.memb_geometry_f64_dev(self.h, ncap, _ptr(x), _ptr(y), _ptr(z),
                                                    _ptr(nx), _ptr(ny), _ptr(nz), _ptr(area)))
 
+    def traction_select_dev(self, sel, x, y, z, gs, eb, fx, fy, fz, nx, ny, nz, area):
+        """The law for the capsules with sel != 0 (int32 device tensor of ncap) only."""
+        ncap = x.numel() // self.npts
+        self._check(self.lib.memb_traction_select_f64_dev(
+            self.h, ncap, _ptr(sel), float(gs), float(eb), _ptr(x), _ptr(y), _ptr(z), _ptr(fx),
+            _ptr(fy), _ptr(fz), _ptr(nx), _ptr(ny), _ptr(nz), _ptr(area)))
+
+    def corot_dev(self, X, x, y, z, delta, dhx, dhy, dhz, rot, amp):
+        ncap = x.numel() // self.npts
+        self._check(self.lib.memb_corot_f64_dev(self.h, ncap, _ptr(X), _ptr(x), _ptr(y), _ptr(z),
+                                                _ptr(delta), _ptr(dhx), _ptr(dhy), _ptr(dhz),
+                                                _ptr(rot), _ptr(amp)))
+
+    def corot_apply_dev(self, flin, resid, rot, amp, fx, fy, fz):
+        ncap = amp.numel()
+        self._check(self.lib.memb_corot_apply_f64_dev(self.h, ncap, _ptr(flin), _ptr(resid),
+                                                      _ptr(rot), _ptr(amp), _ptr(fx), _ptr(fy),
+                                                      _ptr(fz)))
+
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
             self.lib.memb_plan_destroy(self.h)

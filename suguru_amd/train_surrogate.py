@@ -11,7 +11,7 @@ from .surrogate import run_experiment
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n-train", type=int, default=200_000)
-    ap.add_argument("--n-test", type=int, default=2048)
+    ap.add_argument("--n-test", type=int, default=4096)
     ap.add_argument("--steps", type=int, default=20_000)
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -19,10 +19,11 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--log-every", type=int, default=1000)
     ap.add_argument("--save", default=None)
-    ap.add_argument("--amp-max", type=float, default=0.3)
+    ap.add_argument("--shapes", default="flow", choices=["flow", "field"])
     a = ap.parse_args()
-    r = run_experiment(a.n_train, a.n_test, a.steps, a.batch, a.lr, a.seed, a.tol,
-                       log_every=a.log_every, save=a.save, amp_max=a.amp_max)
+    r = run_experiment(n_train=a.n_train, n_test=a.n_test, steps=a.steps, batch=a.batch,
+                       lr=a.lr, seed=a.seed, tol=a.tol, log_every=a.log_every, save=a.save,
+                       kind=a.shapes)
     print(json.dumps(r), flush=True)
 
 

@@ -12,7 +12,11 @@ directly.  Everything else is pinned by what the mathematics fixes:
   bending contribution (uniform moment); zero net force and torque on deformed shapes
   (equilibrium of a closed membrane); virtual work: the traction equals minus the gradient of
   the neo-Hookean energy; rotation covariance on the grid's own symmetry (one longitude step);
-  translation invariance; linearity in the moduli.
+  translation invariance; linearity in the moduli;
+* the bending branch: the linear response of the unit sphere to r = 1 + eps Y_lm is radial,
+  mode-diagonal, independent of m and equal to a (l-1) l (l+1) (l+2) eps Y_lm with one a > 0
+  for l = 2..8 (the isotropic fourth-order operator with the dilation and translation null
+  modes).
 """
 import ctypes
 import os
@@ -269,3 +273,42 @@ def test_linear_in_moduli(lib, grid):
     f3 = law(lib, x, y, z, gs=2.0, eb=3.0)[:3]
     for a, b, c in zip(f1, f2, f3):
         assert np.allclose(c, 2 * a + 3 * b, rtol=1e-12, atol=1e-12 * np.abs(c).max())
+
+
+def _real_ylm(l, m, TH, PH):
+    Y = sph_harm_y(l, abs(m), TH, PH)
+    return (np.sqrt(2) * (Y.imag if m < 0 else Y.real)) if m != 0 else Y.real
+
+
+def test_bending_linear_response_of_the_sphere(lib, grid):
+    """The bending branch (eb != 0, gs = 0) about the unit sphere, pinned by symmetry: for the
+    radial perturbation r = 1 + eps Y_lm the radial part of the load is -kappa_l eps Y_lm
+    (mode-diagonal; kappa independent of m by isotropy), and for a fourth-order isotropic
+    surface operator kappa is quadratic in lambda = l (l + 1) with roots at l = 0 (a dilated
+    sphere carries no bending load: the Laplace test) and l = 1 (translation), so
+    kappa_l = a (l - 1) l (l + 1) (l + 2) with a > 0 (the load restores the sphere).  A
+    sign flip of the moment, a transposed index or a dropped curvature-derivative term breaks
+    one of these."""
+    th, w, TH, PH = grid
+    wq = (w[:, None] * np.full((1, NP), 2 * np.pi / NP)).ravel()
+    X = np.stack([(np.sin(TH) * np.sin(PH)).ravel(), (np.sin(TH) * np.cos(PH)).ravel(),
+                  np.cos(TH).ravel()])
+    eps, eb = 1e-5, 1.0
+    a_l = []
+    for l in range(2, 9):
+        kap = []
+        for m in range(-l, l + 1):
+            Y = _real_ylm(l, m, TH, PH)
+            fp = np.stack(law(lib, *shape(TH, PH, 1.0 + eps * Y), gs=0.0, eb=eb)[:3])
+            fm = np.stack(law(lib, *shape(TH, PH, 1.0 - eps * Y), gs=0.0, eb=eb)[:3])
+            q = ((fp - fm) / (2 * eps) * X).sum(0)           # radial part of d f / d eps
+            y = Y.ravel()
+            k = -(q * y * wq).sum() / (y * y * wq).sum()
+            assert np.abs(q + k * y).max() <= 1e-6 * np.abs(k * y).max(), (l, m)
+            kap.append(k)
+        kap = np.array(kap)
+        assert np.abs(kap - kap.mean()).max() <= 1e-7 * abs(kap.mean()), (l, kap)
+        a_l.append(kap.mean() / ((l - 1) * l * (l + 1) * (l + 2)))
+    a_l = np.array(a_l)
+    assert np.all(a_l > 0)
+    assert np.abs(a_l - a_l.mean()).max() <= 1e-6 * a_l.mean(), a_l

@@ -7,10 +7,12 @@ times with HIP events between its stages (`eager_ms_per_step`); those events giv
 per-kernel durations of the roofline.  Default workload: the
 512 x 256 x 512 channel (h = 1/64, x,z periodic, y walls) with 2000 seeded non-overlapping
 capsules of radius U(4h,10h) and degree 8..24 by radius (1.16 M Lagrangian points).
-`--workload dense`: one GPU's share of the dense 1024 x 512 x 256 channel, 128 x 512 x 256 cells
-with 1000 capsules of radius 10.6h and degree 16 at 30 % volume fraction (578 000 points).
+`--workload dense-full`: the dense coupled channel at full size, 1024 x 512 x 256 cells with
+8000 capsules of radius 10.6h and degree 16 at 30 % volume fraction (4 624 000 points);
+`--workload dense`: its 1/8 share, 128 x 512 x 256 cells with 1000 of those capsules.
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--workload uneven|dense|moderate|small]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5]
+                    [--workload uneven|dense-full|dense|moderate|small]
 
 Prints one JSON line.  `value` = Lagrangian points per second over interp + spread.
 `roofline` is for the dominant kernel, the spread's main kernel (k_obj_spread for the
@@ -305,7 +307,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="uneven", choices=["uneven", "dense", "moderate", "small"])
+    ap.add_argument("--workload", default="uneven",
+                    choices=["uneven", "dense", "dense-full", "moderate", "small"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-graph", action="store_true", help="eager launches instead of a graph")
     ap.add_argument("--config", type=int, default=-1,
@@ -352,8 +355,8 @@ def main():
     from suguru_amd import Batch, IBTransfer, workload as W
 
     t_setup = time.perf_counter()
-    maker = {"uneven": W.config_uneven, "dense": W.config_dense_share, "moderate": W.config_moderate,
-             "small": W.config_small}
+    maker = {"uneven": W.config_uneven, "dense": W.config_dense_share, "dense-full": W.config_dense_full,
+             "moderate": W.config_moderate, "small": W.config_small}
     wl = maker[args.workload]()
     g, caps = wl.grid, wl.caps
     dev = torch.device("cuda:0")

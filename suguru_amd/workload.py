@@ -18,6 +18,7 @@ x = xc + R sin(theta) sin(phi), y = yc + R sin(theta) cos(phi), z = zc + R cos(t
 from __future__ import annotations
 
 import dataclasses
+import functools
 import math
 
 import numpy as np
@@ -75,16 +76,20 @@ def grid_from_channel(nx: int, ny: int, nz: int, h: float = H_DEFAULT) -> Grid:
 # ---------------------------------------------------------------------------------------
 
 
+@functools.lru_cache(maxsize=None)
 def sphere_layout(degree: int):
     """Unit-sphere Gauss grid for spherical-harmonic degree L: nlat = L+1, nlon = 2 nlat.
 
-    Returns (theta[nlat] north->south, phi[nlon], gauss weights w[nlat])."""
+    Returns (theta[nlat] north->south, phi[nlon], gauss weights w[nlat]), read-only (cached
+    per degree)."""
     nlat = degree + 1
     nlon = 2 * nlat
     xg, wg = np.polynomial.legendre.leggauss(nlat)  # ascending cos(theta)
     theta = np.arccos(xg[::-1])  # north (theta ~ 0) to south
     w = wg[::-1].copy()
     phi = np.arange(nlon, dtype=np.float64) * (2.0 * math.pi / nlon)
+    for a in (theta, phi, w):
+        a.setflags(write=False)
     return theta, phi, w
 
 

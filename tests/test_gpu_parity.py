@@ -312,3 +312,101 @@ def test_device_kernel_sweep_and_kerneltest(gpu_ctx_factory):
     s = ctypes.c_double()
     assert lib.ibx_kerneltest_dev(ctx.h, ctypes.byref(s)) == 0
     assert s.value == float("0.99999999999996703") == float(gold["kerneltest"][0])
+
+
+def test_full_size_dense_channel_subset_bitexact_and_identities(gpu_ctx_factory):
+    """The dense coupled channel at full size: 1024 x 512 x 256 cells, 8000 capsules of
+    degree 16 (4.62 M points) at 30 %.  A 40-capsule subset is bitwise against the oracle on
+    the full grid; the whole batch runs through the batched path (capsule-object spread,
+    captured hipGraph), where conservation, adjointness, replay-to-replay determinism and
+    graph = eager are checked on the device."""
+    import torch
+    from suguru_amd import Batch
+
+    wl = W.config_dense_full(with_fields=False)
+    g, caps = wl.grid, wl.caps
+    print("case: dense 1024x512x256, 8000 capsules", flush=True)
+    gen = torch.Generator(device="cuda:0")
+    gen.manual_seed(17)
+    nf = g.ncells_halo
+    dev = "cuda:0"
+    t = {k: torch.randn(nf, dtype=torch.float64, device=dev, generator=gen) for k in ("u", "v", "w")}
+    host = {k: t[k].cpu().numpy().reshape(g.shape_f, order="F") for k in ("u", "v", "w")}
+    ctx = gpu_ctx_factory(g)
+    sel = np.arange(0, caps.ncap, 200)
+    xs, ys, zs, As, off = [], [], [], [], [0]
+    for p in sel:
+        s = slice(caps.cap_off[p], caps.cap_off[p + 1])
+        xs.append(caps.x[s]), ys.append(caps.y[s]), zs.append(caps.z[s]), As.append(caps.area[s])
+        off.append(off[-1] + xs[-1].size)
+    sub = W.Capsules(np.array(off, dtype=np.int64), np.concatenate(xs), np.concatenate(ys),
+                     np.concatenate(zs), np.concatenate(As), caps.centers[sel], caps.radii[sel],
+                     caps.degrees[sel])
+    fs = W.forces(sub.npts, 8)
+    print("case: dense full, 40-capsule subset vs oracle", flush=True)
+    gi = ctx.eulr2lagr(host["u"], host["v"], host["w"], sub.cap_off, sub.x, sub.y, sub.z)
+    oi = O.eulr2lagr(g, host["u"], host["v"], host["w"], sub.cap_off, sub.x, sub.y, sub.z)
+    for a, b, nm in zip(gi, oi, ("ul", "vl", "wl")):
+        assert_bitwise(a, b, nm)
+    gs = ctx.lagr2eulr(sub.cap_off, sub.x, sub.y, sub.z, *fs, sub.area)
+    os_ = O.lagr2eulr(g, sub.cap_off, sub.x, sub.y, sub.z, *fs, sub.area)
+    for a, b, nm in zip(gs, os_, ("forcex", "forcey", "forcez")):
+        assert_bitwise(a, b, nm)
+    del gs, os_, host
+    # the whole batch on the device
+    print("case: dense full, whole batch, objects-w8 graph", flush=True)
+    up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    f = W.forces(caps.npts, 9)
+    # capsules of the lattice come within 1.1 cells of the walls, where the clamped windows
+    # drop taps: conservation is checked with the forces of those points set to zero
+    zf = caps.z * g.dzi
+    near = (zf < 3.0) | (zf > g.ktot - 3.0)
+    assert near.any() and not near.all()
+    for a in f:
+        a[near] = 0.0
+    for k, a in zip(("fx", "fy", "fz"), f):
+        t[k] = up(a)
+    for k in ("x", "y", "z", "area"):
+        t[k] = up(getattr(caps, k))
+    n = caps.npts
+    for k in ("ul", "vl", "wl"):
+        t[k] = torch.empty(n, dtype=torch.float64, device=dev)
+    for k in ("forcex", "forcey", "forcez"):
+        t[k] = torch.full((nf,), np.nan, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    b = Batch(ctx, caps.cap_off)
+    try:
+        b.bind(**t)
+        b.set_config(Batch.config_names().index("objects-w8"))
+        b.capture(use_graph=True)
+        b.run()
+        ctx.sync()
+        first = [t[k].clone() for k in ("ul", "vl", "wl", "forcex", "forcey", "forcez")]
+        dV = g.dx * g.dy * g.dz
+        shp = (g.ktot + 2, g.jtot + 2, g.itot + 2)
+        inner = lambda a: a.view(shp)[1:-1, 1:-1, 1:-1]
+        for comp, (vel, fl, l, F) in enumerate(zip(("u", "v", "w"), ("fx", "fy", "fz"),
+                                                   ("ul", "vl", "wl"),
+                                                   ("forcex", "forcey", "forcez"))):
+            fa = t[fl] * t["area"]
+            tot = float(fa.sum())
+            assert abs(float(inner(t[F]).sum()) * dV - tot) < 1e-11 * float(fa.abs().sum()), comp
+            lhs = float((inner(t[vel]) * inner(t[F])).sum()) * dV
+            rhs = float((t[l] * fa).sum())
+            assert abs(lhs - rhs) < 1e-11 * float(t[vel].abs().max()) * float(fa.abs().sum()), comp
+        for rep in range(2):
+            for k in ("ul", "vl", "wl", "forcex", "forcey", "forcez"):
+                t[k].fill_(np.nan)
+            torch.cuda.synchronize()
+            b.run()
+            ctx.sync()
+            for k, a in zip(("ul", "vl", "wl", "forcex", "forcey", "forcez"), first):
+                assert torch.equal(t[k].view(torch.int64), a.view(torch.int64)), (rep, k)
+        print("case: dense full, eager batch", flush=True)
+        b.capture(use_graph=False)
+        b.run()
+        ctx.sync()
+        for k, a in zip(("ul", "vl", "wl", "forcex", "forcey", "forcez"), first):
+            assert torch.equal(t[k].view(torch.int64), a.view(torch.int64)), ("eager", k)
+    finally:
+        b.close()

@@ -17,6 +17,10 @@
  *   correc(p)               correc.f90:11-38   | fld_correc_dev
  *   the sequence            main.f90:388-410   | fld_step_dev
  *   chkdiv                  chkdiv.f90:7-61    | fld_chkdiv_dev
+ *   indicator (ifoverlap=0) indicator.f90:47-168 | fld_indicator_dev (after the spread of
+ *                                              |   the normals, ibx_spread_cc_f64_dev)
+ *     divergence of G       indicator.f90:47-67  | fld_indicator_div_dev
+ *     boundind .. forces    indicator.f90:91-168 | fld_indicator_finish_dev
  *
  * Fields are the reference's real(8) (0:i1,0:j1,0:k1) arrays (common.f90:6-9), i fastest,
  * in device memory.  The grid is the transfer library's ibx_grid (include/ibx.h); itot and
@@ -87,6 +91,34 @@ int fld_step_dev(fld_ctx* ctx, double* unew, double* vnew, double* wnew, double*
                  double* p, const double* uo, const double* vo, const double* wo,
                  const double* uoo, const double* voo, const double* woo,
                  const double* forcex, const double* forcey, const double* forcez);
+/* The indicator function's divergence, indicator.f90:47-67: the interior of indi =
+ * (gx(i+1)-gx(i-1)) .5 dxi + (gy(j+1)-gy(j-1)) .5 dyi + (gz(k+1)-gz(k-1)) .5 dzi, reading the
+ * halos of G as lagr2eulrind leaves them (x and y periodic copies, k halos with their own
+ * contributions).  Bit for bit the reference's expression. */
+int fld_indicator_div_dev(fld_ctx* ctx, const double* gx, const double* gy, const double* gz,
+                          double* indi);
+/* The indicator after its Poisson solve, indicator.f90:91-168 (ifoverlap = 0):
+ * boundind(indi) (= boundp, bound.f90:113-139); indic(k) = indi(k + ktot/4) for
+ * k <= kmax - ktot/4, else 0; boundind(indic); on the interior indic < .01 -> 0,
+ * indic >= .9 -> 1; visc = visc0 (1 + (lambda - 1) indic), boundp(visc);
+ * forcey += .5 (indic(j) + indic(j+1)) sin_t tcoef, forcez += .5 (indic(k) + indic(k+1))
+ * (-cos_t) tcoef, with sin_t = sin(thetainc), cos_t = cos(thetainc), tcoef =
+ * (ta / Rep^2)^.25 (param.f90:52,57,59 folded on the host).  The halos of indic keep their
+ * values from before the thresholds, as in the reference.  Bit for bit the reference's
+ * expressions. */
+int fld_indicator_finish_dev(fld_ctx* ctx, double* indi, double* indic, double* visc,
+                             double* forcey, double* forcez, double lambda, double visc0,
+                             double sin_t, double cos_t, double tcoef);
+/* The indicator function, indicator.f90:47-168 (ifoverlap = 0): fld_indicator_div_dev,
+ * the Poisson solve of initsolver(0) / solver1d(indi, 0) (the pressure solve, Neumann walls),
+ * fld_indicator_finish_dev.  G = (gx, gy, gz) is the cell-centred spread of the normals with
+ * the points shifted up by (ktot/4) dz (indicator.f90:25-43; ibx_shift_z_dev and
+ * ibx_spread_cc_f64_dev).  The solve uses this library's FFTs: indi agrees with the reference
+ * to rounding, and a cell whose value lies within rounding of .01 or .9 may threshold the
+ * other way. */
+int fld_indicator_dev(fld_ctx* ctx, const double* gx, const double* gy, const double* gz,
+                      double* indi, double* indic, double* visc, double* forcey, double* forcez,
+                      double lambda, double visc0, double sin_t, double cos_t, double tcoef);
 /* chkdiv, chkdiv.f90:7-61: total and largest absolute discrete divergence of the interior
  * (host outputs; waits for the stream).  The total is summed in a different order than the
  * reference's sequential loop. */

@@ -107,6 +107,22 @@ int ibx_kernel_dev(ibx_ctx* ctx, int64_t n, const double* r, double* out);
  * the reference's order (k outer, i inner); waits and writes the sum to *sum (host). */
 int ibx_kerneltest_dev(ibx_ctx* ctx, double* sum);
 
+/* ---- wall collisions and the indicator's position shift -------------------------------- */
+
+/* collisions, collisions.f90:10-32: fz[q] += the Morse force of the bottom wall on point q,
+ * sum over the wall-plane lattice j, k in [-jm, jm], jm = nint(sqrt(max(rad0^2 - z^2, 0))
+ * dyi), of a2 (exp(2 betacol (rad0 - s)) - exp(betacol (rad0 - s))) |z / s|
+ * min(area, dx^2) / max(area, dx^2) over the lattice distances s < rad0.  a2 = 2 col / Rep^2
+ * (collisions.f90:24), rad0 = 2.5 dz and betacol from param.f90:64.  Device arrays of npts;
+ * asynchronous.  The exponentials are the device's, so results agree with a host evaluation
+ * to rounding (~1e-15 relative), not bit for bit.  rad0 must span at most 64 cells. */
+int ibx_collisions_dev(ibx_ctx* ctx, int64_t npts, const double* z, const double* area,
+                       double* fz, double a2, double rad0, double betacol);
+/* z[q] = z[q] + s in place, q < npts: the indicator's shift of ap(p)%zfp up by (ktot/4) dz
+ * and back (indicator.f90:25-31, 37-43), so that positions return as (z + s) - s exactly as
+ * in the reference.  Asynchronous. */
+int ibx_shift_z_dev(ibx_ctx* ctx, int64_t npts, double* z, double s);
+
 /* ---- host-array entry points (the Fortran drop-in calls these) -------------------------- */
 
 /* eulr2lagr, interp_spread.f90:12-475.  u,v,w: fields (0:i1,0:j1,0:k1).  x,y,z: positions of

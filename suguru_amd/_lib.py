@@ -101,6 +101,8 @@ def load() -> ctypes.CDLL:
         "ibx_kernel": (D, [D]),
         "ibx_kernel_dev": (ctypes.c_int, [P, I64, P, P]),
         "ibx_kerneltest_dev": (ctypes.c_int, [P, ctypes.POINTER(D)]),
+        "ibx_collisions_dev": (ctypes.c_int, [P, I64, P, P, P, D, D, D]),
+        "ibx_shift_z_dev": (ctypes.c_int, [P, I64, P, D]),
         "ibx_eulr2lagr": (ctypes.c_int, [P, P, P, P, I64, P, P, P, P, P, P, P]),
         "ibx_lagr2eulr": (ctypes.c_int, [P, I64, P, P, P, P, P, P, P, P, P, P, P]),
         "ibx_lagr2eulrind": (ctypes.c_int, [P, I64, P, P, P, P, P, P, P, P, P, P, P]),
@@ -239,6 +241,9 @@ def load_fluid() -> ctypes.CDLL:
         "fld_correc_dev": (I, [P, P, P, P, P]),
         "fld_step_dev": (I, [P] + [P] * 14),
         "fld_chkdiv_dev": (I, [P, P, P, P, ctypes.POINTER(D), ctypes.POINTER(D)]),
+        "fld_indicator_div_dev": (I, [P, P, P, P, P]),
+        "fld_indicator_finish_dev": (I, [P] + [P] * 5 + [D] * 5),
+        "fld_indicator_dev": (I, [P] + [P] * 8 + [D] * 5),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -7,12 +7,18 @@ Per step the reference runs (main.f90:188-410, fsolver = "f", no overlap mesh):
     -> (surf_vol) -> fftuvw -> bounduvw -> fillps -> solver1d(p) -> boundp -> correc
     -> bounduvw -> pnew += p -> boundp(pnew)
 
-The dense configuration leaves out collisions, the indicator and surf_vol (diagnostics).
-Every stage is enqueued on one HIP stream, with no host round trip:
+surf_vol is a diagnostic and is left out.  The dense configuration (physics="dense") also
+leaves out the wall collisions and the indicator function; physics="full" runs them in their
+places.  Every stage is enqueued on one HIP stream, with no host round trip:
 
     1. ibx_interp_f64_dev          velocity at every point             (eulr2lagr)
     2. traction                    the FP64 law (memb_traction_f64_dev) or a surrogate
+       [ibx_collisions_dev         wall Morse force added to fz        (collisions.f90:10-32)]
     3. ibx_spread_f64_dev_noalloc  traction x area onto the grid       (lagr2eulr)
+       [indicator                  ibx_shift_z_dev(+s), ibx_spread_cc_f64_dev of the normals,
+                                   ibx_shift_z_dev(-s), fld_indicator_dev: indicator function,
+                                   viscosity, centrifugal buoyancy added to forcey / forcez
+                                   (indicator.f90:19-182, s = (ktot/4) dz)]
     4. ibx_update_pos_dev          x += u dt, centroid, periodic shift (Update_Pos.f90:77-186)
     5. fld_step_dev                the flow update                     (main.f90:388-410)
 
@@ -31,9 +37,7 @@ import numpy as np
 from .fluid import DT, REP, FlowSolver
 from .interp_spread import IBTransfer, _offsets
 from .membrane import MembranePlan
-
-GSCELL = 6250.0         # 1 / We, We = Rep * Ca = 0.0016 * 0.1 (param.f90:57,88)
-EBEND = 0.3 * GSCELL    # cbend * gscell (param.f90:91)
+from .params import EBEND, GSCELL  # 1 / We (folded: 6249.999999999999) and 0.3 gscell
 
 
 class CoupledStep:
@@ -48,8 +52,11 @@ class CoupledStep:
 
     def __init__(self, grid, cap_off, degree: int = 16, gs: float = GSCELL, eb: float = EBEND,
                  rep: float = REP, dt: float = DT, device: int = 0, spread_config="objects-w8",
-                 traction=None):
+                 traction=None, physics: str = "dense"):
         import torch
+
+        if physics not in ("dense", "full"):
+            raise ValueError("physics must be 'dense' or 'full'")
 
         self.torch = torch
         off = _offsets(cap_off)
@@ -85,6 +92,25 @@ class CoupledStep:
         self.ctx.set_stream(s)
         self.plan.set_stream(s)
         self.fluid.set_stream(s)
+        self.physics = physics
+        self.stage_names = ("interp", "traction", "spread", "update_pos", "fluid")
+        if physics == "full":
+            from . import params
+
+            self.stage_names = ("interp", "traction", "collisions", "spread", "indicator",
+                                "update_pos", "fluid")
+            self.G = [e(nf) for _ in range(3)]
+            self.indi, self.indic, self.visc = e(nf), e(nf), e(nf)
+            self.a2 = params.collision_factor(rep)
+            self.betacol = params.BETACOL
+            self.rad0 = params.rad0(grid.dz)
+            self.zshift = (grid.ktot // 4) * grid.dz
+            # the normals' spread sizes its work buffers by the capsule count alone: one call
+            # now, so that no allocation falls inside a later graph capture
+            self.ctx.spread_dev(self.off, self.x, self.y, self.z, self.nx, self.ny, self.nz,
+                                self.area, *self.G, cc=True)
+            self.stream.synchronize()
+            self.ctx.sync()
         self.phase = 0          # which velocity set is unew
         self.graphs = None
         self.steps_done = 0
@@ -128,30 +154,48 @@ class CoupledStep:
 
     def _enqueue(self, phase, events=None):
         unew, uo, uoo = self._sets(phase)
-        mark = (lambda i: events[i].record(self.stream)) if events is not None else (lambda i: None)
+        n = [0]
+
+        def mark():
+            if events is not None:
+                events[n[0]].record(self.stream)
+            n[0] += 1
+
+        full = self.physics == "full"
         u, v, w = uo  # the velocity of this step: unew before the rotation (main.f90:225)
-        mark(0)
+        mark()
         self.ctx.interp_dev(u, v, w, self.off, self.x, self.y, self.z, self.ul, self.vl, self.wl)
-        mark(1)
+        mark()
         if self.traction is None:
             self.plan.traction_dev(self.x, self.y, self.z, self.gs, self.eb, self.fx, self.fy,
                                    self.fz, self.nx, self.ny, self.nz, self.area)
         else:
             self.traction.enqueue(self.x, self.y, self.z, self.fx, self.fy, self.fz, self.nx,
                                   self.ny, self.nz, self.area)
-        mark(2)
+        mark()
+        if full:
+            self.ctx.collisions_dev(self.z, self.area, self.fz, self.a2, self.rad0, self.betacol)
+            mark()
         self.ctx.spread_dev(self.off, self.x, self.y, self.z, self.fx, self.fy, self.fz,
                             self.area, *self.force, noalloc=True)
-        mark(3)
+        mark()
+        if full:
+            self.ctx.shift_z_dev(self.z, self.zshift)
+            self.ctx.spread_dev(self.off, self.x, self.y, self.z, self.nx, self.ny, self.nz,
+                                self.area, *self.G, cc=True)
+            self.ctx.shift_z_dev(self.z, -self.zshift)
+            self.fluid.indicator(*self.G, self.indi, self.indic, self.visc, self.force[1],
+                                 self.force[2])
+            mark()
         self.ctx.update_pos_dev(self.off, self.x, self.y, self.z, self.ul, self.vl, self.wl,
                                 self.dt, self.centroid, self.umean)
-        mark(4)
+        mark()
         self.fluid.step(*unew, self.pnew, self.p, *uo, *uoo, *self.force)
-        mark(5)
+        mark()
 
     def step(self, events=None):
-        """One eager step (asynchronous on the step's stream).  events: six torch.cuda.Event
-        recorded before, between and after the five stages."""
+        """One eager step (asynchronous on the step's stream).  events: len(stage_names) + 1
+        torch.cuda.Event recorded before, between and after the stages."""
         self._enqueue(self.phase, events)
         self.phase = (self.phase + 2) % 3
         self.steps_done += 1

@@ -469,6 +469,83 @@ __global__ void __launch_bounds__(256) k_bound(FGrid g, double* __restrict__ f0,
   }
 }
 
+// indicator.f90:51-67: the central divergence of G = (gx, gy, gz), ((g(+1) - g(-1)) .5) d?i
+// summed x + y + z, for every interior cell, into the compact array cp (the right-hand side
+// of the indicator's Poisson solve) or, with f != nullptr, into the interior of the field f.
+__global__ void __launch_bounds__(256) k_ind_div(FGrid g, const double* __restrict__ gx,
+                                                 const double* __restrict__ gy,
+                                                 const double* __restrict__ gz,
+                                                 double* __restrict__ cp,
+                                                 double* __restrict__ f) {
+  const long long n = (long long)g.itot * g.jtot * g.ktot;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.itot) + 1;
+  const long long r = t / g.itot;
+  const int j = (int)(r % g.jtot) + 1, k = (int)(r / g.jtot) + 1;
+  const long long q = fidx(g, i, j, k);
+  const double dgx = (gx[q + 1] - gx[q - 1]) * .5 * g.dxi;
+  const double dgy = (gy[q + g.sy] - gy[q - g.sy]) * .5 * g.dyi;
+  const double dgz = (gz[q + g.sz] - gz[q - g.sz]) * .5 * g.dzi;
+  const double dg = dgx + dgy + dgz;
+  if (f)
+    f[q] = dg;
+  else
+    cp[t] = dg;
+}
+
+// indicator.f90:93-103: indic(i,j,k) = indi(i,j,k + ktot/4) for k <= kmax - ktot/4, else 0
+// (the solution moved back down by the planes the capsules were shifted up)
+__global__ void __launch_bounds__(256) k_ind_shift(FGrid g, const double* __restrict__ indi,
+                                                   double* __restrict__ indic) {
+  const long long n = (long long)g.itot * g.jtot * g.ktot;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.itot) + 1;
+  const long long r = t / g.itot;
+  const int j = (int)(r % g.jtot) + 1, k = (int)(r / g.jtot) + 1;
+  const int kq = g.ktot / 4;
+  indic[fidx(g, i, j, k)] = k <= g.ktot - kq ? indi[fidx(g, i, j, k + kq)] : 0.0;
+}
+
+// indicator.f90:105-112: < .01 -> 0, >= .9 -> 1 (idempotent)
+__device__ __forceinline__ double ind_threshold(double v) {
+  if (v < .01) v = 0.;
+  if (v >= .9) v = 1.;
+  return v;
+}
+
+// indicator.f90:105-168 after boundind(indic), fused per interior cell: the thresholds, the
+// viscosity visc0 (1 + (lambda - 1) indic) (:137-143) and the centrifugal buoyancy
+//   forcey += .5 (indic(j) + indic(j+1)) sin(thetainc) tcoef          (:164)
+//   forcez += .5 (indic(k) + indic(k+1)) (-cos(thetainc)) tcoef       (:165)
+// The reference thresholds the whole interior before the force loop, so an interior
+// neighbour is read thresholded and a halo neighbour (j+1 = j1, k+1 = k1) as boundind left
+// it.  A neighbour is read here and thresholded on the spot: the threshold is idempotent, so
+// reading it before or after its own thread rewrote it gives the same value.
+__global__ void __launch_bounds__(256) k_ind_finish(FGrid g, double* __restrict__ indic,
+                                                    double* __restrict__ visc,
+                                                    double* __restrict__ forcey,
+                                                    double* __restrict__ forcez, double lambda,
+                                                    double visc0, double sin_t, double mcos_t,
+                                                    double tcoef) {
+  const long long n = (long long)g.itot * g.jtot * g.ktot;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int i = (int)(t % g.itot) + 1;
+  const long long r = t / g.itot;
+  const int j = (int)(r % g.jtot) + 1, k = (int)(r / g.jtot) + 1;
+  const long long q = fidx(g, i, j, k);
+  const double v = ind_threshold(indic[q]);
+  double vy = indic[q + g.sy], vz = indic[q + g.sz];
+  if (j < g.jtot) vy = ind_threshold(vy);
+  if (k < g.ktot) vz = ind_threshold(vz);
+  indic[q] = v;
+  visc[q] = visc0 * (1. + ((lambda - 1.) * v));
+  forcey[q] = forcey[q] + (.5 * (v + vy) * sin_t * tcoef);
+  forcez[q] = forcez[q] + (.5 * (v + vz) * mcos_t * tcoef);
+}
+
 // chkdiv.f90:7-61: per-block sums and maxima of the discrete divergence
 __global__ void __launch_bounds__(256) k_div(FGrid g, const double* __restrict__ u,
                                              const double* __restrict__ v,
@@ -888,6 +965,65 @@ int fld_step_dev(fld_ctx* c, double* unew, double* vnew, double* wnew, double* p
   k_addp<<<nblocks(c->ncomp, 256), 256, 0, c->stream>>>(c->g, pnew, p);
   HIPF(c, hipGetLastError());
   return bound(c, pnew, nullptr, nullptr, 1, 2);
+}
+
+int fld_indicator_div_dev(fld_ctx* c, const double* gx, const double* gy, const double* gz,
+                          double* indi) {
+  if (!c) return FLD_E_ARG;
+  if (!(gx && gy && gz && indi)) return fail(c, FLD_E_ARG, "null field");
+  HIPF(c, hipSetDevice(c->device));
+  k_ind_div<<<nblocks(c->ncomp, 256), 256, 0, c->stream>>>(c->g, gx, gy, gz, nullptr, indi);
+  HIPF(c, hipGetLastError());
+  return FLD_OK;
+}
+
+static int ind_finish(fld_ctx* c, double* indi, double* indic, double* visc, double* forcey,
+                      double* forcez, double lambda, double visc0, double sin_t, double cos_t,
+                      double tcoef) {
+  int rc;
+  if ((rc = bound(c, indi, nullptr, nullptr, 1, 2))) return rc;  // boundind(indi)
+  k_ind_shift<<<nblocks(c->ncomp, 256), 256, 0, c->stream>>>(c->g, indi, indic);
+  HIPF(c, hipGetLastError());
+  if ((rc = bound(c, indic, nullptr, nullptr, 1, 2))) return rc;  // boundind(indic)
+  k_ind_finish<<<nblocks(c->ncomp, 256), 256, 0, c->stream>>>(c->g, indic, visc, forcey, forcez,
+                                                              lambda, visc0, sin_t, -cos_t,
+                                                              tcoef);
+  HIPF(c, hipGetLastError());
+  return bound(c, visc, nullptr, nullptr, 1, 2);  // boundp(visc)
+}
+
+static bool ind_args_ok(double lambda, double visc0, double sin_t, double cos_t, double tcoef) {
+  return std::isfinite(lambda) && std::isfinite(visc0) && std::isfinite(sin_t) &&
+         std::isfinite(cos_t) && std::isfinite(tcoef);
+}
+
+int fld_indicator_finish_dev(fld_ctx* c, double* indi, double* indic, double* visc,
+                             double* forcey, double* forcez, double lambda, double visc0,
+                             double sin_t, double cos_t, double tcoef) {
+  if (!c) return FLD_E_ARG;
+  if (!(indi && indic && visc && forcey && forcez)) return fail(c, FLD_E_ARG, "null field");
+  if (!ind_args_ok(lambda, visc0, sin_t, cos_t, tcoef))
+    return fail(c, FLD_E_ARG, "non-finite indicator constant");
+  HIPF(c, hipSetDevice(c->device));
+  return ind_finish(c, indi, indic, visc, forcey, forcez, lambda, visc0, sin_t, cos_t, tcoef);
+}
+
+int fld_indicator_dev(fld_ctx* c, const double* gx, const double* gy, const double* gz,
+                      double* indi, double* indic, double* visc, double* forcey, double* forcez,
+                      double lambda, double visc0, double sin_t, double cos_t, double tcoef) {
+  if (!c) return FLD_E_ARG;
+  if (!(gx && gy && gz && indi && indic && visc && forcey && forcez))
+    return fail(c, FLD_E_ARG, "null field");
+  if (!ind_args_ok(lambda, visc0, sin_t, cos_t, tcoef))
+    return fail(c, FLD_E_ARG, "non-finite indicator constant");
+  HIPF(c, hipSetDevice(c->device));
+  // divergence straight into the solver's compact array, solve (initsolver(0) and
+  // solver1d(indi, 0) are the pressure solve, ne = 1), solution into the interior of indi
+  k_ind_div<<<nblocks(c->ncomp, 256), 256, 0, c->stream>>>(c->g, gx, gy, gz, c->work[0], nullptr);
+  HIPF(c, hipGetLastError());
+  int rc = solve_compact(c, c->work[0], 1, indi);
+  if (rc) return rc;
+  return ind_finish(c, indi, indic, visc, forcey, forcez, lambda, visc0, sin_t, cos_t, tcoef);
 }
 
 int fld_chkdiv_dev(fld_ctx* c, const double* u, const double* v, const double* w,

@@ -578,6 +578,34 @@ int ibx_kerneltest_dev(ibx_ctx* c, double* sum) {
   return IBX_OK;
 }
 
+int ibx_collisions_dev(ibx_ctx* c, int64_t npts, const double* z, const double* area,
+                       double* fz, double a2, double rad0, double betacol) {
+  if (!c) return IBX_E_ARG;
+  if (npts < 0) return fail(c, IBX_E_ARG, "negative size");
+  if (npts > 0 && !(z && area && fz)) return fail(c, IBX_E_ARG, "null pointer");
+  if (!(rad0 >= 0.0) || !std::isfinite(a2) || !std::isfinite(betacol))
+    return fail(c, IBX_E_ARG, "rad0 must be >= 0, a2 and betacol finite");
+  // the plane sum spans 2 nint(rad0 dyi) + 1 points per direction; keep it a stencil
+  if (rad0 * c->grid.dyi > 64.0) return fail(c, IBX_E_ARG, "rad0 spans more than 64 cells");
+  if (npts == 0) return IBX_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  k_collisions<<<blocks_for(npts, 256), 256, 0, c->stream>>>(c->gd, npts, z, area, fz, c->grid.dx,
+                                                             c->grid.dy, a2, rad0, betacol);
+  HIPCHK(c, hipGetLastError());
+  return IBX_OK;
+}
+
+int ibx_shift_z_dev(ibx_ctx* c, int64_t npts, double* z, double s) {
+  if (!c) return IBX_E_ARG;
+  if (npts < 0) return fail(c, IBX_E_ARG, "negative size");
+  if (npts > 0 && !z) return fail(c, IBX_E_ARG, "null pointer");
+  if (npts == 0) return IBX_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  k_shift_z<<<blocks_for(npts, 256), 256, 0, c->stream>>>(npts, z, s);
+  HIPCHK(c, hipGetLastError());
+  return IBX_OK;
+}
+
 int ibx_create(ibx_ctx** out, const ibx_grid* g, int device) {
   if (!out || !g) return IBX_E_ARG;
   *out = nullptr;

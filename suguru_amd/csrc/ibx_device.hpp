@@ -1814,6 +1814,49 @@ __global__ void __launch_bounds__(256) k_advance(GridD g, double lz, double dt, 
   x[q] = xq, y[q] = yq, z[q] = zq;
 }
 
+// collisions.f90:10-32: the Morse wall force of the bottom wall added to fz of every point.
+// a2 = 2 col / Rep^2, rad0 the cutoff and zero-force distance, betacol the Morse width.
+// The plane sum over j, k in [-jm, jm], jm = nint(sqrt(max(rad0^2 - z^2, 0)) dyi), in the
+// reference's loop order (j outer, k inner; j steps dy, k steps dx) and operation order.
+// Non-finite z gives jm = 0 and no term (the comparison ss < rad0 fails).
+__global__ void __launch_bounds__(256) k_collisions(GridD g, long long npts,
+                                                    const double* __restrict__ z,
+                                                    const double* __restrict__ area,
+                                                    double* __restrict__ fz, double dx,
+                                                    double dy, double a2, double rad0,
+                                                    double betacol) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= npts) return;
+  const double zq = z[q];
+  const double r2 = rad0 * rad0 - zq * zq;
+  const double rj = sqrt(r2 > 0.0 ? r2 : 0.0) * g.dyi;
+  const int jm = (int)round(rj);  // Fortran nint: half away from zero
+  const double a = area[q], dx2 = dx * dx;
+  const double lo = a < dx2 ? a : dx2, hi = a > dx2 ? a : dx2;
+  double fc = 0.0;
+  for (int j = -jm; j <= jm; ++j) {
+    const double jy = (double)j * dy;
+    for (int k = -jm; k <= jm; ++k) {
+      const double kx = (double)k * dx;
+      const double ss = sqrt(zq * zq + jy * jy + kx * kx);
+      if (ss < rad0) {
+        const double e2 = exp(2.0 * betacol * (rad0 - ss));
+        const double e1 = exp(betacol * (rad0 - ss));
+        fc = fc + a2 * (e2 - e1) * fabs(zq / ss) * (lo / hi);
+      }
+    }
+  }
+  fz[q] = fz[q] + fc;
+}
+
+// z[q] = z[q] + s for every point: the in-place shift of ap(p)%zfp around the indicator's
+// spread (indicator.f90:25-31 with s = (ktot/4) dz, :37-43 with -s).
+__global__ void __launch_bounds__(256) k_shift_z(long long npts, double* __restrict__ z,
+                                                 double s) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < npts) z[q] = z[q] + s;
+}
+
 // One workgroup per capsule: the 256 threads stage 256 consecutive points at a time in LDS
 // (coalesced loads), and thread c < 6 adds component c of them in point order, so the sums
 // are the reference's sequential ones (:135-145); then the shift decision and the shift.

@@ -14,8 +14,18 @@ import ctypes
 from . import _lib
 from .interp_spread import _dev_check, _ptr
 
-REP = 0.0016  # particle Reynolds number (param.f90:57)
+REP = 0.0016  # particle Reynolds number (param.f90:57; suguru_amd.params.REP)
 DT = 1e-4     # time step (main.f90:119, 491)
+
+
+def _ind_consts(lambda_, visc0, sin_t, cos_t, tcoef):
+    """The indicator's constants, each defaulting to param.f90's value."""
+    from . import params
+
+    st, ct, tc = params.centrifugal()
+    pick = lambda v, d: float(d if v is None else v)
+    return (pick(lambda_, params.LAMBDA), pick(visc0, params.VISC0), pick(sin_t, st),
+            pick(cos_t, ct), pick(tcoef, tc))
 
 
 class FluidError(RuntimeError):
@@ -92,6 +102,28 @@ class FlowSolver:
         ptrs = self._fields(unew=unew, vnew=vnew, wnew=wnew, pnew=pnew, p=p, uo=uo, vo=vo, wo=wo,
                             uoo=uoo, voo=voo, woo=woo, forcex=fx, forcey=fy, forcez=fz)
         self._check(self.lib.fld_step_dev(self.h, *ptrs))
+
+    def indicator_div(self, gx, gy, gz, indi):
+        """indicator.f90:47-67: the interior of indi = divergence of G."""
+        self._check(self.lib.fld_indicator_div_dev(self.h, *self._fields(gx=gx, gy=gy, gz=gz,
+                                                                         indi=indi)))
+
+    def indicator_finish(self, indi, indic, visc, forcey, forcez, lambda_=None, visc0=None,
+                         sin_t=None, cos_t=None, tcoef=None):
+        """indicator.f90:91-168 after the solve (constants default to param.f90's)."""
+        k = _ind_consts(lambda_, visc0, sin_t, cos_t, tcoef)
+        self._check(self.lib.fld_indicator_finish_dev(
+            self.h, *self._fields(indi=indi, indic=indic, visc=visc, forcey=forcey,
+                                  forcez=forcez), *k))
+
+    def indicator(self, gx, gy, gz, indi, indic, visc, forcey, forcez, lambda_=None,
+                  visc0=None, sin_t=None, cos_t=None, tcoef=None):
+        """indicator.f90:47-168: divergence, Poisson solve, shift, thresholds, viscosity and
+        centrifugal buoyancy; G is the cell-centred spread of the normals at z + (ktot/4) dz."""
+        k = _ind_consts(lambda_, visc0, sin_t, cos_t, tcoef)
+        self._check(self.lib.fld_indicator_dev(
+            self.h, *self._fields(gx=gx, gy=gy, gz=gz, indi=indi, indic=indic, visc=visc,
+                                  forcey=forcey, forcez=forcez), *k))
 
     def chkdiv(self, u, v, w):
         """(total, largest absolute) discrete divergence of the interior; waits."""

@@ -218,6 +218,20 @@ class IBTransfer:
                        _ptr(y), _ptr(z), _ptr(fx), _ptr(fy), _ptr(fz), _ptr(area),
                        _ptr(forcex), _ptr(forcey), _ptr(forcez)))
 
+    def collisions_dev(self, z, area, fz, a2, rad0, betacol):
+        """collisions.f90:10-32 on device tensors: fz += the bottom wall's Morse force.
+        a2 = 2 col / Rep^2, rad0 = 2.5 dz, betacol (param.f90:64; suguru_amd.params)."""
+        n = int(z.numel())
+        for nm, t in (("z", z), ("area", area), ("fz", fz)):
+            _dev_check(nm, t, n)
+        self._check(self.lib.ibx_collisions_dev(self.h, n, _ptr(z), _ptr(area), _ptr(fz),
+                                                float(a2), float(rad0), float(betacol)))
+
+    def shift_z_dev(self, z, s):
+        """z += s in place (the indicator's shift of the points, indicator.f90:25-43)."""
+        _dev_check("z", z, int(z.numel()))
+        self._check(self.lib.ibx_shift_z_dev(self.h, int(z.numel()), _ptr(z), float(s)))
+
     def update_pos_dev(self, cap_off, x, y, z, ul, vl, wl, dt, centroid, umean, xold=None,
                        yold=None):
         n, ncap = int(x.numel()), int(cap_off.numel()) - 1

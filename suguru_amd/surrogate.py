@@ -24,7 +24,7 @@ captured in a hipGraph.
 Shapes for training and evaluation: the unit sphere displaced by a random vector field of
 spherical-harmonic degree 1..8 in all three components (coefficients ~ N(0,1)/n), scaled to an
 RMS amplitude drawn log-uniformly, then rotated and translated at random.  The law's moduli:
-gscell = 1/We = 6250, ebend = 0.3 gscell (param.f90:57,88,91).
+gscell = 1/We (6249.999999999999 in FP64), ebend = 0.3 gscell (param.f90:57,88,91).
 """
 from __future__ import annotations
 
@@ -35,9 +35,7 @@ import time
 import numpy as np
 
 from .membrane import MembranePlan, SurrogateMLP
-
-GSCELL = 6250.0         # 1 / We, We = Rep * Ca = 0.0016 * 0.1 (param.f90:57,88)
-EBEND = 0.3 * GSCELL    # cbend * gscell (param.f90:91)
+from .params import EBEND, GSCELL  # 1 / We (folded: 6249.999999999999) and 0.3 gscell
 
 
 def grid_angles(L: int):

@@ -225,3 +225,52 @@ def chkdiv(grid, u, v, w):
     t, m = ctypes.c_double(), ctypes.c_double()
     _flib().ora_chkdiv(ctypes.byref(gs), _p(u), _p(v), _p(w), ctypes.byref(t), ctypes.byref(m))
     return t.value, m.value
+
+
+# ---- wall collisions and the indicator function (oracle/indicator_oracle.c) ----------------
+def _ilib():
+    o = _flib()
+    if not getattr(o, "_ind_sig", False):
+        G = ctypes.POINTER(IbxGrid)
+        D = ctypes.c_double
+        o.ora_collisions.argtypes = [G, ctypes.c_int64, P, P, P, D, D, D]
+        o.ora_indicator_div.argtypes = [G, P, P, P, P]
+        o.ora_indicator_finish.argtypes = [G] + [P] * 5 + [D] * 5
+        o.ora_indicator.argtypes = [G, D, D, ctypes.c_int64] + [P] * 16 + [D] * 5
+        o._ind_sig = True
+    return o
+
+
+def collisions(grid, z, area, fz, a2, rad0, betacol):
+    """collisions.f90:10-32: adds the wall force to fz in place."""
+    gs = grid_struct(grid)
+    _ilib().ora_collisions(ctypes.byref(gs), z.size, _p(z), _p(area), _p(fz), a2, rad0, betacol)
+
+
+def indicator_div(grid, gx, gy, gz):
+    indi = grid.empty_field()
+    gs = grid_struct(grid)
+    _ilib().ora_indicator_div(ctypes.byref(gs), _p(gx), _p(gy), _p(gz), _p(indi))
+    return indi
+
+
+def indicator_finish(grid, indi, forcey, forcez, lambda_, visc0, sin_t, cos_t, tcoef):
+    """In place on indi (halos), forcey, forcez; returns (indic, visc)."""
+    indic, visc = grid.empty_field(), grid.empty_field()
+    gs = grid_struct(grid)
+    _ilib().ora_indicator_finish(ctypes.byref(gs), _p(indi), _p(indic), _p(visc), _p(forcey),
+                                 _p(forcez), lambda_, visc0, sin_t, cos_t, tcoef)
+    return indic, visc
+
+
+def indicator(grid, rep, dt, off, x, y, z, xn, yn, zn, area, forcey, forcez, lambda_, visc0,
+              sin_t, cos_t, tcoef):
+    """indicator.f90:19-182; z (shifted and back), forcey and forcez in place.  Returns
+    (gx, gy, gz, indi, indic, visc)."""
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    out = [grid.empty_field() for _ in range(6)]
+    gs = grid_struct(grid)
+    _ilib().ora_indicator(ctypes.byref(gs), rep, dt, off.size - 1, _p(off), _p(x), _p(y), _p(z),
+                          _p(xn), _p(yn), _p(zn), _p(area), *[_p(a) for a in out], _p(forcey),
+                          _p(forcez), lambda_, visc0, sin_t, cos_t, tcoef)
+    return tuple(out)

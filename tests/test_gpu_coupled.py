@@ -16,6 +16,7 @@ import numpy as np
 import pytest
 
 import _oracle as O
+from suguru_amd import params as PR
 from suguru_amd import workload as W
 
 pytestmark = pytest.mark.gpu
@@ -60,8 +61,9 @@ def oracle_law(L, x, y, z, n):
     return out
 
 
-def oracle_steps(g, caps, vel, L, nsteps):
-    """The reference's step order on the CPU oracle; returns per step (u, v, w, pnew, x, y, z)."""
+def oracle_steps(g, caps, vel, L, nsteps, full=False):
+    """The reference's step order on the CPU oracle; returns per step (u, v, w, pnew, x, y, z).
+    full: with the wall collisions and the indicator function (main.f90:231,237)."""
     n = caps.npts // caps.ncap
     cur = [np.asfortranarray(a.copy()) for a in vel]
     O.bounduvw(g, *cur)
@@ -73,8 +75,14 @@ def oracle_steps(g, caps, vel, L, nsteps):
     for _ in range(nsteps):
         older, old = old, [a.copy(order="F") for a in cur]
         ul, vl, wl = O.eulr2lagr(g, old[0], old[1], old[2], caps.cap_off, x, y, z)
-        fx, fy, fz, _, _, _, area = oracle_law(L, x, y, z, n)
+        fx, fy, fz, nx, ny, nz, area = oracle_law(L, x, y, z, n)
+        if full:
+            O.collisions(g, z, area, fz, PR.collision_factor(REP), PR.rad0(g.dz), PR.BETACOL)
         F = O.lagr2eulr(g, caps.cap_off, x, y, z, fx, fy, fz, area)
+        if full:
+            st, ct, tc = PR.centrifugal()
+            O.indicator(g, REP, DT, caps.cap_off, x, y, z, nx, ny, nz, area, F[1], F[2],
+                        PR.LAMBDA, PR.VISC0, st, ct, tc)
         O.update_pos(g, caps.cap_off, DT, x, y, z, ul, vl, wl)
         O.fluid_step(g, REP, DT, *cur, pn, *old, *older, *F)
         out.append(([a.copy(order="F") for a in cur], pn.copy(order="F"), x.copy(), y.copy(), z.copy()))
@@ -132,6 +140,53 @@ def test_graph_replays_match_eager_steps():
             st.run()
             st.sync()
             # eager torch work between the replays
+            junk = torch.randn(4096, device="cuda:0", dtype=torch.float64).sum()
+            torch.cuda.synchronize()
+            assert torch.isfinite(junk)
+            got = state(st, g)
+            for a, b, nm in zip(got[0] + (got[1],) + got[2:], eager[k][0] + (eager[k][1],) + eager[k][2:],
+                                ("u", "v", "w", "pnew", "x", "y", "z")):
+                assert np.array_equal(bits(a), bits(b)), (k, nm)
+    finally:
+        st.close()
+
+
+def test_full_physics_steps_match_oracle_chain():
+    """physics="full": the wall collisions and the indicator function in the reference's
+    places (collisions after the traction, the indicator after the spread), one capsule close
+    enough to the bottom wall for the Morse force (rad0 = 2.5 dz), three steps against the
+    oracle chain.  Also every graph replay against the eager steps."""
+    torch = _torch()
+    from suguru_amd.coupled_step import CoupledStep
+
+    g = W.make_grid(32, 16, 16, 0.25)
+    caps = W.build_capsules([(0.5, 2.1, 1.45), (4.6, 1.8, 2.2)], [1.0, 1.0], [8, 8])
+    vel = W.velocity_field(g, 23, nmodes=4)
+    assert caps.z.min() < PR.rad0(g.dz)
+    print("case: coupled full physics 32x16x16, 2 capsules deg 8, 3 steps vs oracle", flush=True)
+    want = oracle_steps(g, caps, vel, 8, 3, full=True)
+    st = CoupledStep(g, caps.cap_off, degree=8, gs=GS, eb=EB, rep=REP, dt=DT, physics="full")
+    try:
+        st.load(*vel, caps.x, caps.y, caps.z)
+        eager = []
+        for k in range(3):
+            st.step()
+            st.sync()
+            got = state(st, g)
+            eager.append(got)
+            (u, v, w), pn, x, y, z = got
+            (wu, wv, ww), wpn, wx, wy, wz = want[k]
+            for a, b, nm in ((u, wu, "u"), (v, wv, "v"), (w, ww, "w"), (pn, wpn, "pnew")):
+                err = np.abs(a - b).max()
+                assert err <= 1e-8 * np.abs(b).max(), (k, nm, err)
+            for a, b, nm in ((x, wx, "x"), (y, wy, "y"), (z, wz, "z")):
+                assert np.abs(a - b).max() <= 1e-12 * g.lx, (k, nm)
+        print("case: coupled full physics, graph capture and 3 replays", flush=True)
+        st.load(*vel, caps.x, caps.y, caps.z)
+        st.capture()
+        for k in range(3):
+            st.run()
+            st.sync()
             junk = torch.randn(4096, device="cuda:0", dtype=torch.float64).sum()
             torch.cuda.synchronize()
             assert torch.isfinite(junk)

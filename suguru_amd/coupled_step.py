@@ -135,6 +135,7 @@ class CoupledStep:
             for dst, a in zip((self.x, self.y, self.z), (x, y, z)):
                 dst.copy_(up(a))
         self.phase = 0
+        self._x0 = tuple(np.asarray(a, dtype=np.float64).copy() for a in (x, y, z))
         self.fluid.bounduvw(*self.vel[0])
         self.fluid.boundp(self.pnew)
         self.sync()
@@ -219,6 +220,100 @@ class CoupledStep:
             self.graphs[self.phase].replay()
         self.phase = (self.phase + 2) % 3
         self.steps_done += 1
+
+    # ---- output and restart files (suguru_amd.formats) --------------------------------------
+    def _interior(self, t):
+        g = self.grid
+        a = t.view(g.ktot + 2, g.jtot + 2, g.itot + 2)[1:-1, 1:-1, 1:-1]
+        return a.cpu().numpy().transpose(2, 1, 0)  # (itot, jtot, ktot), Fortran order
+
+    def write_output(self, datadir: str, istep: int, time: float, dt=None):
+        """The text output of main.f90:455-489 after step istep: Capsule###step#######.txt of
+        every capsule, the center_velocity.txt lines (one per capsule: the first capsule's
+        centroid and the last capsule's mean point velocity, as Update_Pos leaves them) and the
+        timestep.txt line.  Waits for the stream."""
+        import os
+
+        from . import formats as F
+
+        self.sync()
+        n = self.npts // self.ncap
+        x, y, z = (a.cpu().numpy().reshape(self.ncap, n) for a in (self.x, self.y, self.z))
+        for p in range(self.ncap):
+            F.write_capsule_txt(os.path.join(datadir, F.capsule_txt_name(p + 1, istep)), x[p],
+                                y[p], z[p])
+        cen = self.centroid.view(-1, 3).cpu().numpy()
+        um = self.umean.view(-1, 3).cpu().numpy()
+        F.append_center_velocity(os.path.join(datadir, "center_velocity.txt"), time, cen[0],
+                                 um[-1], lines=self.ncap)
+        F.append_timestep(os.path.join(datadir, "timestep.txt"), istep, time,
+                          self.dt if dt is None else dt)
+
+    def save_restart(self, datadir: str, istep: int, time: float):
+        """fldbg####### (unew, vnew, wnew, uo, vo, wo, pnew and time, istep, dt) and
+        allpartdata####### (per capsule: centroid, points, velocities, forces, areas; the
+        initial points of the last load as xfp0; integralv, volume, volume0, dxdt.. and the
+        older point sets 0: this step keeps none of them), loadd.f90:84-103, 588-666.  Waits."""
+        import os
+
+        from . import formats as F
+
+        self.sync()
+        unew, uo = self.vel[self.phase], self.vel[(self.phase + 1) % 3]
+        fields = {"unew": unew[0], "vnew": unew[1], "wnew": unew[2], "uo": uo[0], "vo": uo[1],
+                  "wo": uo[2], "pnew": self.pnew}
+        F.write_fldbg(os.path.join(datadir, F.fldbg_name(istep)),
+                      {k: self._interior(t) for k, t in fields.items()}, time, istep, self.dt)
+        n = self.npts // self.ncap
+        rec = np.zeros(self.ncap, dtype=F.particle_record_dtype(n))
+        cen = self.centroid.view(-1, 3).cpu().numpy()
+        rec["x"], rec["y"], rec["z"] = cen[:, 0], cen[:, 1], cen[:, 2]
+        pts = {"xfp": self.x, "yfp": self.y, "zfp": self.z, "ul": self.ul, "vl": self.vl,
+               "wl": self.wl, "fxl": self.fx, "fyl": self.fy, "fzl": self.fz, "area": self.area}
+        for k, t in pts.items():
+            rec[k] = t.cpu().numpy().reshape(self.ncap, n)
+        if getattr(self, "_x0", None) is not None:
+            for k, a in zip(("xfp0", "yfp0", "zfp0"), self._x0):
+                rec[k] = a.reshape(self.ncap, n)
+        F.write_allpartdata(os.path.join(datadir, F.allpartdata_name(istep)), rec)
+
+    def load_restart(self, datadir: str, istep: int):
+        """Reads fldbg####### and allpartdata####### (loadflds(0) / loadpart(0)): unew and uo
+        interiors with their halos rebuilt by bounduvw, pnew with boundp, the capsule points;
+        uoo is not stored (the next step makes it uo).  Returns (time, istep, dt)."""
+        import os
+
+        from . import formats as F
+
+        g, t = self.grid, self.torch
+        fields, time, nr, dt = F.read_fldbg(os.path.join(datadir, F.fldbg_name(istep)), g.itot,
+                                            g.jtot, g.ktot)
+        n = self.npts // self.ncap
+        rec = F.read_allpartdata(os.path.join(datadir, F.allpartdata_name(istep)), self.ncap, n)
+
+        def put(dst, a):  # interior (itot, jtot, ktot) into a halo field
+            v = dst.view(g.ktot + 2, g.jtot + 2, g.itot + 2)
+            v[1:-1, 1:-1, 1:-1] = t.from_numpy(np.ascontiguousarray(a.transpose(2, 1, 0))).to(self.dev)
+
+        with t.cuda.stream(self.stream):
+            for st in range(3):
+                for k in range(3):
+                    self.vel[st][k].zero_()
+            for k, nm in enumerate(("unew", "vnew", "wnew")):
+                put(self.vel[0][k], fields[nm])
+            for k, nm in enumerate(("uo", "vo", "wo")):
+                put(self.vel[1][k], fields[nm])
+            self.pnew.zero_()
+            put(self.pnew, fields["pnew"])
+            self.p.zero_()
+            for dst, nm in zip((self.x, self.y, self.z), ("xfp", "yfp", "zfp")):
+                dst.copy_(t.from_numpy(np.ascontiguousarray(rec[nm].reshape(-1))).to(self.dev))
+        self.phase = 0
+        self.fluid.bounduvw(*self.vel[0])
+        self.fluid.bounduvw(*self.vel[1])
+        self.fluid.boundp(self.pnew)
+        self.sync()
+        return time, nr, dt
 
     def sync(self):
         """Waits for the stream and raises the first error any library flagged."""

@@ -196,3 +196,48 @@ def test_full_physics_steps_match_oracle_chain():
                 assert np.array_equal(bits(a), bits(b)), (k, nm)
     finally:
         st.close()
+
+
+def test_restart_continues_bit_for_bit(tmp_path):
+    """save_restart after two steps (fldbg / allpartdata, loadd.f90), a new CoupledStep
+    load_restart, two more steps: the same bits as four uninterrupted steps.  write_output's
+    text files hold the points (7 digits) and one center_velocity line per capsule."""
+    _torch()
+    from suguru_amd import formats as F
+    from suguru_amd.coupled_step import CoupledStep
+
+    g, caps, vel = small_case()
+    print("case: coupled restart, 4 steps vs 2 + restart + 2", flush=True)
+    st = CoupledStep(g, caps.cap_off, degree=8, gs=GS, eb=EB, rep=REP, dt=DT)
+    try:
+        st.load(*vel, caps.x, caps.y, caps.z)
+        for _ in range(4):
+            st.step()
+        st.sync()
+        want = state(st, g)
+        st.load(*vel, caps.x, caps.y, caps.z)
+        for _ in range(2):
+            st.step()
+        st.save_restart(str(tmp_path), 2, 2 * DT)
+        st.write_output(str(tmp_path), 2, 2 * DT)
+        x2 = st.x.cpu().numpy()
+    finally:
+        st.close()
+    n = caps.npts // caps.ncap
+    for p in range(caps.ncap):
+        rx, ry, rz = F.read_capsule_txt(str(tmp_path / F.capsule_txt_name(p + 1, 2)))
+        assert np.allclose(rx, x2[p * n:(p + 1) * n], rtol=1e-6, atol=0)
+    assert len((tmp_path / "center_velocity.txt").read_text().splitlines()) == caps.ncap
+    st2 = CoupledStep(g, caps.cap_off, degree=8, gs=GS, eb=EB, rep=REP, dt=DT)
+    try:
+        time, nr, dt = st2.load_restart(str(tmp_path), 2)
+        assert (time, nr, dt) == (2 * DT, 2, DT)
+        for _ in range(2):
+            st2.step()
+        st2.sync()
+        got = state(st2, g)
+        for a, b, nm in zip(got[0] + (got[1],) + got[2:], want[0] + (want[1],) + want[2:],
+                            ("u", "v", "w", "pnew", "x", "y", "z")):
+            assert np.array_equal(bits(a), bits(b)), nm
+    finally:
+        st2.close()

@@ -274,3 +274,51 @@ def indicator(grid, rep, dt, off, x, y, z, xn, yn, zn, area, forcey, forcez, lam
                           _p(xn), _p(yn), _p(zn), _p(area), *[_p(a) for a in out], _p(forcey),
                           _p(forcez), lambda_, visc0, sin_t, cos_t, tcoef)
     return tuple(out)
+
+
+# ---- the Fortran port of the transfer loops (oracle/ibx_port.f90, CPU baseline) ------------
+PORT_VARIANTS = ("O0", "O2", "O0_omp", "O2_omp")
+_ports = {}
+
+
+def port_lib(variant):
+    """oracle/_port/libibxport_<variant>.so (built by oracle/Makefile), or None if absent."""
+    if variant not in _ports:
+        path = os.path.join(ROOT, "oracle", "_port", f"libibxport_{variant}.so")
+        if not os.path.exists(path):
+            return None
+        L = ctypes.CDLL(path)
+        D, I, I64 = ctypes.c_double, ctypes.c_int, ctypes.c_int64
+        L.port_eulr2lagr.argtypes = [I, I, I, D, D, D, D, D, P, P, P, I64, P, P, P, P, P, P, P]
+        L.port_lagr2eulr.argtypes = [I, I, I, D, D, D, D, D, D, I64, P, P, P, P, P, P, P, P, P,
+                                     P, P]
+        L.port_max_threads.restype = I
+        L.port_set_threads.argtypes = [I]
+        _ports[variant] = L
+    return _ports[variant]
+
+
+def port_eulr2lagr(variant, grid, u, v, w, off, x, y, z, threads=1):
+    L = port_lib(variant)
+    L.port_set_threads(int(threads))
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    n = int(off[-1])
+    out = [np.zeros(n) for _ in range(3)]
+    c = np.ascontiguousarray
+    L.port_eulr2lagr(grid.itot, grid.jtot, grid.ktot, grid.lx, grid.ly, grid.dxi, grid.dyi,
+                     grid.dzi, _p(u), _p(v), _p(w), off.size - 1, _p(off), _p(c(x)), _p(c(y)),
+                     _p(c(z)), *[_p(a) for a in out])
+    return tuple(out)
+
+
+def port_lagr2eulr(variant, grid, off, x, y, z, fx, fy, fz, area, threads=1, out=None):
+    L = port_lib(variant)
+    L.port_set_threads(int(threads))
+    off = np.ascontiguousarray(off, dtype=np.int64)
+    o = out if out is not None else [grid.empty_field() for _ in range(3)]
+    c = np.ascontiguousarray
+    L.port_lagr2eulr(grid.itot, grid.jtot, grid.ktot, grid.lx, grid.ly, grid.dxi, grid.dyi,
+                     grid.dzi, grid.dx * grid.dy * grid.dz, off.size - 1, _p(off), _p(c(x)),
+                     _p(c(y)), _p(c(z)), _p(c(fx)), _p(c(fy)), _p(c(fz)), _p(c(area)),
+                     *[_p(a) for a in o])
+    return tuple(o)

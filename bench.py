@@ -40,6 +40,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, vendor figure
+_T0 = time.perf_counter()
+
+
+def progress(msg: str) -> None:
+    """A progress line on stderr (the JSON result line alone goes to stdout)."""
+    print(f"bench [{time.perf_counter() - _T0:7.1f} s]: {msg}", file=sys.stderr, flush=True)
 
 
 def support_cells(g, caps):
@@ -230,6 +236,7 @@ def coupled_leg(torch, steps=100, warmup=2, surrogate=True):
         a = time.perf_counter()
         sur = Surrogate(L=16, max_batch=8192)
         sur.build_jacobian()
+        progress("coupled: surrogate dataset and training")
         X, Y, _, _, _ = sur.dataset(200_000, 1, kind="flow")
         sur.fit_norm(X, Y)
         sur.mlp.init(1)
@@ -294,11 +301,13 @@ def coupled_leg(torch, steps=100, warmup=2, surrogate=True):
             st.traction = None
             st.close()
 
+    progress("coupled: FP64 law run")
     plain = run("law")
     out["plain"] = {k: v for k, v in plain.items() if k in ("graph_ms_per_step", "stages_us_eager")}
     x0 = np.stack([caps.centers[:, 0], caps.centers[:, 1], caps.centers[:, 2]], 1)
     if surrogate:
         for kind in ("surrogate", "linear_anchored"):
+            progress(f"coupled: {kind} run")
             r = run(kind)
             disp_p = plain["centroid"] - x0
             disp_s = r["centroid"] - x0
@@ -544,22 +553,27 @@ def main():
         },
         "setup_s": round(t_setup, 2),
     }
+    progress(f"transfer line done ({out['value'] / 1e9:.3f} G pts/s)")
     if not args.no_surrogate:
+        progress("surrogate leg")
         try:
             out["surrogate"] = surrogate_leg()
         except Exception as e:  # reported, never hides the transfer line
             out["surrogate"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_capsule_step:
+        progress("capsule-step leg")
         try:
             out["capsule_step"] = capsule_step_leg(torch)
         except Exception as e:  # reported, never hides the transfer line
             out["capsule_step"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_coupled:
+        progress("coupled-step leg")
         try:
             out["coupled_step"] = coupled_leg(torch, steps=args.coupled_steps)
         except Exception as e:  # reported, never hides the transfer line
             out["coupled_step"] = {"error": f"{type(e).__name__}: {e}"}
     if not args.no_cpu:
+        progress("CPU baseline")
         out["cpu_baseline"] = cpu_baseline(wl)
         if "value" in out["cpu_baseline"]:
             out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]

@@ -403,7 +403,8 @@ def main():
         return
 
     if args.coupled_only:
-        print(json.dumps({"coupled_step": coupled_leg(torch, steps=args.coupled_steps)}))
+        print(json.dumps({"coupled_step": coupled_leg(torch, steps=args.coupled_steps,
+                                                      surrogate=not args.no_surrogate)}))
         return
 
     from suguru_amd import Batch, IBTransfer, workload as W

@@ -204,7 +204,7 @@ def capsule_step_leg(torch, steps=20, warmup=3):
     return {"workload": wl.name + " at h = 1/10.6 (unit-radius capsules)", "capsules": caps.ncap,
             "points": caps.npts, "steps": steps, "ms_per_step": total_ms,
             "stages_us_mean": {n: float(stage[:, i].mean()) for i, n in enumerate(names)},
-            "fluid": "not included: frozen velocity field (no flow solver in this build)",
+            "fluid": "not included: frozen velocity field (the flow update is in the coupled_step leg)",
             "spread_config": "objects-w8 (what the autotuner picks for this workload)"}
 
 

@@ -99,6 +99,31 @@ int memb_corot_apply_f64_dev(memb_plan* p, int64_t ncap, const double* flin, con
                              const double* rot, const double* amp, double* fx, double* fy,
                              double* fz);
 
+/* The anchored surrogate's step (no reference counterpart: the learned stand-in of
+ * complagrforces, forcing.f90:11-114).  Rows are [ncap][3][npts] FP64 in the co-rotated frame.
+ * Prediction and gate: c = <delta, delta0> / |delta0|^2 (0 if delta0 = 0), f' = (lin + c^2 m0)
+ * + b (m0 = NULL: lin + b), f = R f' into fx, fy, fz; sel[c] = 0 when beta * rms(delta -
+ * delta0) <= tol and every value of f is finite, else 1; drift (optional, [ncap]) the rms. */
+int memb_anchored_predict_f64_dev(memb_plan* p, int64_t ncap, const double* lin,
+                                  const double* m0, const double* b, const double* delta,
+                                  const double* delta0, const double* rot, double beta,
+                                  double tol, double* fx, double* fy, double* fz, int* sel,
+                                  double* drift);
+/* New anchors for the capsules with sel != 0, after the law has written their traction into
+ * fx, fy, fz: refreshed[c] += 1 (int64); b = R^T f - lin (- m_now), delta0 = delta, m0 = m_now
+ * and s0 = amp (when m_now is given).  slot (optional): m_now has a row only for the capsules
+ * with slot[c] >= 0 (row slot[c]); a selected capsule with slot[c] < 0 keeps its old anchor.
+ * Without slot, m_now (if given) has one row per capsule.  Other capsules are not touched. */
+int memb_anchored_update_f64_dev(memb_plan* p, int64_t ncap, const int* sel, const int* slot,
+                                 const double* rot, const double* fx, const double* fy,
+                                 const double* fz, const double* lin, const double* m_now,
+                                 const double* delta, const double* amp, double* b,
+                                 double* delta0, double* m0, double* s0, int64_t* refreshed);
+/* The first k capsules with sel != 0, in capsule order: idx[0..k) = their indices (padded
+ * with 0), slot[c] = their rank, or -1 for every other capsule (int32 device arrays). */
+int memb_compact_sel_dev(memb_plan* p, int64_t ncap, const int* sel, int k, int* idx,
+                         int* slot);
+
 /* ---- learned surrogate: MLP, BF16 MFMA with FP32 accumulation ----------------------------- */
 /* Layers: din -> w (SiLU) -> w (SiLU) -> ... (nhidden layers of width w) -> dout, linear
  * output.  Weights are FP32 masters; the GEMMs read BF16 copies, refreshed whenever the

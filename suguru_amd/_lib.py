@@ -178,6 +178,9 @@ def load_memb() -> ctypes.CDLL:
         "memb_traction_select_f64_dev": (I, [P, I64, P, D, D] + [P] * 10),
         "memb_corot_f64_dev": (I, [P, I64] + [P] * 10),
         "memb_corot_apply_f64_dev": (I, [P, I64] + [P] * 7),
+        "memb_anchored_predict_f64_dev": (I, [P, I64] + [P] * 6 + [D, D] + [P] * 5),
+        "memb_anchored_update_f64_dev": (I, [P, I64] + [P] * 15),
+        "memb_compact_sel_dev": (I, [P, I64, P, I, P, P]),
         "memb_mlp_create": (I, [ctypes.POINTER(P), I, I, I, I, I, I]),
         "memb_mlp_destroy": (None, [P]),
         "memb_mlp_last_error": (ctypes.c_char_p, [P]),

@@ -873,6 +873,138 @@ __global__ void __launch_bounds__(256)
   fz[q] = R[6] * g[0] + R[7] * g[1] + R[8] * g[2];
 }
 
+// The anchored surrogate's prediction and gate, one workgroup per capsule (rows of 3n values,
+// component-major: [c][3][n]):
+//   cc = <delta, delta0> / |delta0|^2 (0 when delta0 = 0), drift = sqrt(sum (delta - delta0)^2 / n),
+//   f' = (lin + cc^2 m0) + b (m0 = NULL: f' = lin + b), f = R f' into fx, fy, fz,
+//   sel = 1 unless beta drift <= tol and every f is finite; drift_out (optional).
+__global__ void __launch_bounds__(CR_THREADS)
+    k_anch_predict(int n, const double* __restrict__ lin, const double* __restrict__ m0,
+                   const double* __restrict__ b, const double* __restrict__ delta,
+                   const double* __restrict__ delta0, const double* __restrict__ rot,
+                   double beta, double tol, double* __restrict__ fx, double* __restrict__ fy,
+                   double* __restrict__ fz, int* __restrict__ sel,
+                   double* __restrict__ drift_out) {
+  __shared__ double red[3][CR_THREADS / 64];
+  __shared__ int bad;
+  const long long cap = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const long long r0 = cap * 3 * n;
+  if (tid == 0) bad = 0;
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int q = tid; q < 3 * n; q += CR_THREADS) {
+    const double d = delta[r0 + q], d0 = delta0[r0 + q], e = d - d0;
+    v[0] += d * d0, v[1] += d0 * d0, v[2] += e * e;
+  }
+  for (int a = 0; a < 3; ++a) {
+    double t = v[a];
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_down(t, o);
+    if (lane == 0) red[a][wv] = t;
+  }
+  __syncthreads();
+  double sdd0 = 0.0, s00 = 0.0, sdr = 0.0;
+  for (int w = 0; w < CR_THREADS / 64; ++w) sdd0 += red[0][w], s00 += red[1][w], sdr += red[2][w];
+  const double c = s00 > 0.0 ? sdd0 / s00 : 0.0, c2 = c * c;
+  const double drift = sqrt(sdr / (double)n);
+  const double* R = rot + cap * 9;
+  int nonfinite = 0;
+  for (int p = tid; p < n; p += CR_THREADS) {
+    double g[3];
+    for (int a = 0; a < 3; ++a) {
+      const long long o = r0 + (long long)a * n + p;
+      g[a] = (m0 ? lin[o] + c2 * m0[o] : lin[o]) + b[o];
+    }
+    const long long q = cap * n + p;
+    const double ox = R[0] * g[0] + R[1] * g[1] + R[2] * g[2];
+    const double oy = R[3] * g[0] + R[4] * g[1] + R[5] * g[2];
+    const double oz = R[6] * g[0] + R[7] * g[1] + R[8] * g[2];
+    fx[q] = ox, fy[q] = oy, fz[q] = oz;
+    nonfinite |= !(isfinite(ox) && isfinite(oy) && isfinite(oz));
+  }
+  if (nonfinite) atomicOr(&bad, 1);
+  __syncthreads();
+  if (tid == 0) {
+    sel[cap] = (beta * drift <= tol && !bad) ? 0 : 1;
+    if (drift_out) drift_out[cap] = drift;
+  }
+}
+
+// New anchors for the capsules with sel != 0 (the law has just written their traction):
+//   b = R^T f - lin (- m_now), delta0 = delta, m0 = m_now, s0 = amp, refreshed += 1.
+// slot (optional): m_now holds rows only for the capsules with slot[c] >= 0 (row slot[c]); a
+// selected capsule without a row keeps its old anchor (its traction this step is the law's
+// all the same, and it is selected again at the next step).
+__global__ void __launch_bounds__(CR_THREADS)
+    k_anch_update(int n, const int* __restrict__ sel, const int* __restrict__ slot,
+                  const double* __restrict__ rot, const double* __restrict__ fx,
+                  const double* __restrict__ fy, const double* __restrict__ fz,
+                  const double* __restrict__ lin, const double* __restrict__ m_now,
+                  const double* __restrict__ delta, const double* __restrict__ amp,
+                  double* __restrict__ b, double* __restrict__ delta0, double* __restrict__ m0,
+                  double* __restrict__ s0, long long* __restrict__ refreshed) {
+  const long long cap = blockIdx.x;
+  if (sel[cap] == 0) return;
+  const int tid = threadIdx.x;
+  if (tid == 0) refreshed[cap] += 1;
+  const long long row = slot ? (long long)slot[cap] : cap;
+  if (row < 0) return;
+  const long long r0 = cap * 3 * n, m_r0 = row * 3 * n;
+  const double* R = rot + cap * 9;
+  for (int p = tid; p < n; p += CR_THREADS) {
+    const long long q = cap * n + p;
+    const double f0 = fx[q], f1 = fy[q], f2 = fz[q];
+    const double g[3] = {R[0] * f0 + R[3] * f1 + R[6] * f2, R[1] * f0 + R[4] * f1 + R[7] * f2,
+                         R[2] * f0 + R[5] * f1 + R[8] * f2};
+    for (int a = 0; a < 3; ++a) {
+      const long long o = r0 + (long long)a * n + p;
+      double bl = g[a] - lin[o];
+      if (m_now) {
+        const double mn = m_now[m_r0 + (long long)a * n + p];
+        bl = bl - mn;
+        m0[o] = mn;
+      }
+      b[o] = bl;
+      delta0[o] = delta[o];
+    }
+  }
+  if (tid == 0 && m_now && s0) s0[cap] = amp[cap];
+}
+
+// The first k capsules with sel != 0, in capsule order: idx[r] = c (padded with 0) and
+// slot[c] = r, or -1 for the others and for selected capsules beyond k.  One workgroup.
+constexpr int CS_THREADS = 1024;
+__global__ void __launch_bounds__(CS_THREADS)
+    k_compact_sel(long long ncap, const int* __restrict__ sel, int k, int* __restrict__ idx,
+                  int* __restrict__ slot) {
+  __shared__ int wsum[CS_THREADS / 64];
+  __shared__ int base;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (tid == 0) base = 0;
+  for (int r = tid; r < k; r += CS_THREADS) idx[r] = 0;
+  __syncthreads();
+  for (long long c0 = 0; c0 < ncap; c0 += CS_THREADS) {
+    const long long c = c0 + tid;
+    const bool on = c < ncap && sel[c] != 0;
+    const unsigned long long bal = __ballot(on);
+    if (lane == 0) wsum[wv] = __popcll(bal);
+    __syncthreads();
+    int before = 0, total = 0;
+    for (int w = 0; w < CS_THREADS / 64; ++w) {
+      before += w < wv ? wsum[w] : 0;
+      total += wsum[w];
+    }
+    const int r = base + before + __popcll(bal & __lanemask_lt());
+    if (c < ncap) {
+      const bool kept = on && r < k;
+      slot[c] = kept ? r : -1;
+      if (kept) idx[r] = (int)c;
+    }
+    __syncthreads();
+    if (tid == 0) base += total;
+    __syncthreads();
+  }
+}
+
 size_t lds_bytes(const Tables& T) {
   const int N = T.N, NN = N * N;
   size_t work = std::max<size_t>(3 * NN * 2, (size_t)RING * 3 * 5 * N * 2 + (size_t)RING * T.np * 6);
@@ -1077,6 +1209,57 @@ int memb_corot_f64_dev(memb_plan* p, int64_t ncap, const double* X, const double
   MCHK(p, hipSetDevice(p->device));
   k_corot<<<(unsigned)ncap, CR_THREADS, 0, p->stream>>>(p->T.N * p->T.np2, X, x, y, z, delta,
                                                          dhx, dhy, dhz, rot, amp);
+  MCHK(p, hipGetLastError());
+  return MEMB_OK;
+}
+
+int memb_anchored_predict_f64_dev(memb_plan* p, int64_t ncap, const double* lin,
+                                  const double* m0, const double* b, const double* delta,
+                                  const double* delta0, const double* rot, double beta,
+                                  double tol, double* fx, double* fy, double* fz, int* sel,
+                                  double* drift) {
+  if (!p) return MEMB_E_ARG;
+  if (ncap < 0) return fail(p, MEMB_E_ARG, "ncap < 0");
+  if (ncap > 0 && !(lin && b && delta && delta0 && rot && fx && fy && fz && sel))
+    return fail(p, MEMB_E_ARG, "null pointer");
+  if (ncap == 0) return MEMB_OK;
+  MCHK(p, hipSetDevice(p->device));
+  const int n = p->T.N * p->T.np2;
+  k_anch_predict<<<(unsigned)ncap, CR_THREADS, 0, p->stream>>>(n, lin, m0, b, delta, delta0, rot,
+                                                              beta, tol, fx, fy, fz, sel, drift);
+  MCHK(p, hipGetLastError());
+  return MEMB_OK;
+}
+
+int memb_anchored_update_f64_dev(memb_plan* p, int64_t ncap, const int* sel, const int* slot,
+                                 const double* rot, const double* fx, const double* fy,
+                                 const double* fz, const double* lin, const double* m_now,
+                                 const double* delta, const double* amp, double* b,
+                                 double* delta0, double* m0, double* s0, int64_t* refreshed) {
+  if (!p) return MEMB_E_ARG;
+  if (ncap < 0) return fail(p, MEMB_E_ARG, "ncap < 0");
+  if (ncap > 0 && !(sel && rot && fx && fy && fz && lin && delta && amp && b && delta0 &&
+                    refreshed))
+    return fail(p, MEMB_E_ARG, "null pointer");
+  if (m_now && !m0) return fail(p, MEMB_E_ARG, "m_now needs m0");
+  if (ncap == 0) return MEMB_OK;
+  MCHK(p, hipSetDevice(p->device));
+  const int n = p->T.N * p->T.np2;
+  k_anch_update<<<(unsigned)ncap, CR_THREADS, 0, p->stream>>>(n, sel, slot, rot, fx, fy, fz, lin,
+                                                             m_now, delta, amp, b, delta0, m0,
+                                                             s0, (long long*)refreshed);
+  MCHK(p, hipGetLastError());
+  return MEMB_OK;
+}
+
+int memb_compact_sel_dev(memb_plan* p, int64_t ncap, const int* sel, int k, int* idx,
+                         int* slot) {
+  if (!p) return MEMB_E_ARG;
+  if (ncap < 0 || k < 1) return fail(p, MEMB_E_ARG, "ncap < 0 or k < 1");
+  if (!(sel && idx && slot)) return fail(p, MEMB_E_ARG, "null pointer");
+  if (ncap > 0x7fffffffLL) return fail(p, MEMB_E_ARG, "more than 2^31 - 1 capsules");
+  MCHK(p, hipSetDevice(p->device));
+  k_compact_sel<<<1, CS_THREADS, 0, p->stream>>>(ncap, sel, k, idx, slot);
   MCHK(p, hipGetLastError());
   return MEMB_OK;
 }

@@ -495,17 +495,27 @@ class AnchoredTraction:
     device launch on the surrogate's stream: capturable in a hipGraph.
 
     learned = False drops the MLP term (m0 = 0): the same scheme without the learned part.
+    The network runs only for the capsules whose anchor moves (their SH coefficients
+    gathered into max_refresh rows); geometry (normals, areas) runs on a side stream beside
+    the prediction, and the prediction, gate and anchor updates are fused kernels
+    (memb_anchored_predict_f64_dev / memb_anchored_update_f64_dev).
     """
 
     def __init__(self, sur: Surrogate, ncap: int, beta: float, tol: float = 1e-3,
-                 learned: bool = True):
+                 learned: bool = True, max_refresh: int = None):
         t = sur.torch
         self.s, self.ncap, self.beta, self.tol, self.learned = sur, ncap, float(beta), tol, learned
+        # anchors moved per step with the learned term: the network runs on these rows only
+        # (a capsule beyond them gets the law's traction and moves its anchor a step later)
+        if max_refresh is None:
+            max_refresh = ncap if ncap <= 2048 else max(2048, ncap // 4)
+        self.kmax = int(min(max(1, max_refresh), ncap))
         z = lambda *shape: t.zeros(*shape, dtype=t.float64, device=sur.dev)
         self.delta0, self.b, self.m0 = z(ncap, sur.dout), z(ncap, sur.dout), z(ncap, sur.dout)
         self.s0 = z(ncap)
         self.refreshed = t.zeros(ncap, dtype=t.int64, device=sur.dev)   # law calls per capsule
         self.steps = 0
+        self._side = None   # stream of the geometry pass
 
     def reset(self):
         with self.s._streamed():
@@ -513,61 +523,74 @@ class AnchoredTraction:
                 a.zero_()
         self.steps = 0
 
-    def _prediction(self, x, y, z):
-        """delta, rot, amp, J delta + carried learned term + b, and the learned term at the
-        current shape (for new anchors)."""
+    def _linear(self, delta):
+        """J delta over all capsules: one FP64 GEMM.  (In FP32 the traction, a small difference
+        of large stiffness terms, loses up to ~4e-5 of itself; the calibrated gate then becomes
+        five times stricter and the law runs five times as often.)"""
+        return delta @ self.s.JT
+
+    def _geometry(self, x, y, z, nx, ny, nz, area):
+        """Normals and areas on a side stream (the law's geometry pass), concurrent with the
+        prediction; joined before the selective law, which shares the plan's scratch."""
         s, t = self.s, self.s.torch
+        if self._side is None:
+            self._side = t.cuda.Stream(s.dev)
+        main = t.cuda.current_stream(s.dev)
+        self._side.wait_stream(main)
+        s.plan.set_stream(self._side.cuda_stream)
+        try:
+            s.plan.geometry_dev(x, y, z, nx, ny, nz, area)
+        finally:
+            s.plan.set_stream(s.stream_handle)
+
+    def _predict_into(self, x, y, z, fx, fy, fz, sel, drift=None):
+        """corot, J delta, and the fused prediction + gate (memb_anchored_predict_f64_dev)."""
+        s = self.s
         delta, dh, rot, amp = s.corot(x, y, z)
-        lin = delta @ s.JT
-        if self.learned:
-            m_now = (amp * amp)[:, None] * s.predict(s.features(dh)).double()
-            d00 = (self.delta0 * self.delta0).sum(1)
-            c = t.where(d00 > 0, (delta * self.delta0).sum(1) / t.where(d00 > 0, d00, 1.0),
-                        t.zeros_like(d00))
-            f_rot = lin + (c * c)[:, None] * self.m0 + self.b
-        else:
-            m_now = None
-            f_rot = lin + self.b
-        return delta, rot, amp, lin, f_rot, m_now
+        lin = self._linear(delta)
+        s.plan.anchored_predict_dev(lin, self.m0 if self.learned else None, self.b, delta,
+                                    self.delta0, rot, self.beta, self.tol, fx, fy, fz, sel,
+                                    drift)
+        return delta, dh, rot, amp, lin
 
     def predict(self, x, y, z):
         """(f [ncap, 3, npts] in the global frame, rms drift [ncap]) from the current anchors."""
         s, t = self.s, self.s.torch
         n, ncap = s.npts, self.ncap
         with s._streamed():
-            delta, rot, _, _, f_rot, _ = self._prediction(x, y, z)
-            f = t.bmm(rot.view(ncap, 3, 3), f_rot.view(ncap, 3, n))
-            drift = (delta - self.delta0).pow(2).view(ncap, 3, n).sum(1).mean(1).sqrt()
+            f = t.empty(3, ncap * n, dtype=t.float64, device=s.dev)
+            sel = t.empty(ncap, dtype=t.int32, device=s.dev)
+            drift = t.empty(ncap, dtype=t.float64, device=s.dev)
+            self._predict_into(x, y, z, f[0], f[1], f[2], sel, drift)
+            f = f.view(3, ncap, n).permute(1, 0, 2)
         return f, drift
 
     def enqueue(self, x, y, z, fx, fy, fz, nx, ny, nz, area):
         """Traction, normals and areas of every capsule on the surrogate's stream.  Returns
         the int32 mask of the capsules the law recomputed this step (device)."""
         s, t = self.s, self.s.torch
-        n, ncap = s.npts, self.ncap
         with s._streamed():
-            delta, rot, amp, lin, f_rot, m_now = self._prediction(x, y, z)
-            R = rot.view(ncap, 3, 3)
-            f = t.bmm(R, f_rot.view(ncap, 3, n))  # f = R f'
-            fx.view(ncap, n).copy_(f[:, 0])
-            fy.view(ncap, n).copy_(f[:, 1])
-            fz.view(ncap, n).copy_(f[:, 2])
-            s.plan.geometry_dev(x, y, z, nx, ny, nz, area)
-            drift = (delta - self.delta0).pow(2).view(ncap, 3, n).sum(1).mean(1).sqrt()
-            keep = (self.beta * drift <= self.tol) & t.isfinite(f).view(ncap, -1).all(dim=1)
-            sel = (~keep).to(t.int32)
+            self._geometry(x, y, z, nx, ny, nz, area)
+            sel = t.empty(self.ncap, dtype=t.int32, device=s.dev)
+            delta, dh, rot, amp, lin = self._predict_into(x, y, z, fx, fy, fz, sel)
+            t.cuda.current_stream(s.dev).wait_stream(self._side)
             s.plan.traction_select_dev(sel, x, y, z, s.gs, s.eb, fx, fy, fz, nx, ny, nz, area)
-            # new anchors for the recomputed capsules
-            fl = t.stack([fx.view(ncap, n), fy.view(ncap, n), fz.view(ncap, n)], 1)
-            bl = t.bmm(R.transpose(1, 2).contiguous(), fl).view(ncap, -1) - lin
-            upd = (~keep)[:, None]
-            if m_now is not None:
-                bl = bl - m_now
-                self.m0.copy_(t.where(upd, m_now, self.m0))
-                self.s0.copy_(t.where(~keep, amp, self.s0))
-            self.b.copy_(t.where(upd, bl, self.b))
-            self.delta0.copy_(t.where(upd, delta, self.delta0))
-            self.refreshed.add_(sel.to(t.int64))
+            if self.learned:
+                # the network's term for the (first kmax) capsules whose anchor moves
+                idx = t.empty(self.kmax, dtype=t.int32, device=s.dev)
+                slot = t.empty(self.ncap, dtype=t.int32, device=s.dev)
+                s.plan.compact_sel_dev(sel, self.kmax, idx, slot)
+                il = idx.long()
+                dh_k = [d.view(self.ncap, -1)[il].reshape(-1) for d in dh]
+                a_k = amp[il]
+                m_k = (a_k * a_k)[:, None] * s.predict(s.features(dh_k)).double()
+                s.plan.anchored_update_dev(sel, slot, rot, fx, fy, fz, lin, m_k, delta, amp,
+                                           self.b, self.delta0, self.m0, self.s0,
+                                           self.refreshed)
+            else:
+                s.plan.anchored_update_dev(sel, None, rot, fx, fy, fz, lin, None, delta, amp,
+                                           self.b, self.delta0, self.m0, self.s0,
+                                           self.refreshed)
         self.steps += 1
         return sel
 
@@ -612,7 +635,8 @@ def calibrate_anchored(sur: Surrogate, learned=True, n=4096, seed=77, q=0.99, **
     t = sur.torch
     P0, P1 = anchored_pairs(sur.L, n, seed, **kw)
     up = lambda A, k: sur.upload(np.ascontiguousarray(A[:, k]))
-    at = AnchoredTraction(sur, n, beta=float("inf"), learned=learned)   # inf: every capsule anchors
+    at = AnchoredTraction(sur, n, beta=float("inf"), learned=learned,
+                          max_refresh=n)   # inf: every capsule anchors
     e = lambda: t.empty(n * sur.npts, dtype=t.float64, device=sur.dev)
     outs = [e() for _ in range(7)]
     at.enqueue(up(P0, 0), up(P0, 1), up(P0, 2), *outs)

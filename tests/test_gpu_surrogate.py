@@ -138,9 +138,10 @@ def test_anchored_stage_is_the_law_at_its_anchors_and_gates(trained):
     assert torch.equal(got[~kept], ref[~kept])   # recomputed capsules: the law's own bits
 
 
-def test_gate_fails_closed_on_nonfinite_predictions(trained, monkeypatch):
-    """A prediction row holding NaN or Inf is recomputed by the law at any tolerance, even
-    beta = 0 (which would keep every finite prediction)."""
+def test_gate_fails_closed_on_nonfinite_predictions(trained):
+    """A capsule whose prediction holds NaN or Inf (planted in its anchor offset b) is
+    recomputed by the law at any tolerance, even beta = 0 (which would keep every finite
+    prediction), and its anchor becomes finite again."""
     torch = _torch()
     from suguru_amd.surrogate import AnchoredTraction
 
@@ -148,16 +149,8 @@ def test_gate_fails_closed_on_nonfinite_predictions(trained, monkeypatch):
     x, y, z = _xyz(Pt)
     ncap = x.numel() // s.npts
     at = AnchoredTraction(s, ncap, beta=0.0, tol=1e-3)
-    real = at._prediction
-
-    def broken(x_, y_, z_):
-        d, r, a, lin, f_rot, m = real(x_, y_, z_)
-        f_rot = f_rot.clone()
-        f_rot[3] = float("nan")
-        f_rot[17, 5] = float("inf")
-        return d, r, a, lin, f_rot, m
-
-    monkeypatch.setattr(at, "_prediction", broken)
+    at.b[3] = float("nan")
+    at.b[17, 5] = float("inf")
     e = lambda: torch.empty(x.numel(), dtype=torch.float64, device=s.dev)
     outs = [e() for _ in range(7)]
     sel = at.enqueue(x, y, z, *outs)
@@ -167,6 +160,7 @@ def test_gate_fails_closed_on_nonfinite_predictions(trained, monkeypatch):
         sl = slice(c * s.npts, (c + 1) * s.npts)
         for k in range(3):
             assert torch.equal(outs[k][sl], fl[k][sl])
+    assert bool(torch.isfinite(at.b).all())
 
 
 def test_anchored_stage_replays_in_a_graph(trained):

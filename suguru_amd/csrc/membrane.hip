@@ -1005,6 +1005,115 @@ __global__ void __launch_bounds__(CS_THREADS)
   }
 }
 
+// Geometry only (unit normal and point area on the force grid) with the arithmetic of
+// k_mem<3> -- analysis of the three coordinates, x_theta and x_phi synthesised from the coarse
+// coefficients, n = (x_t x x_p) / |x_t x x_p|, area = |x_t x x_p| w_i / sin(theta_i) dphi
+// (forcing.f90:91; sph.f90:131) -- laid out for throughput rather than for the law's
+// registers: 256 threads per capsule, the longitude tables staged in LDS once per workgroup,
+// the six (i, m) partial sums of x_theta and x_phi in one pass, and one thread per node for
+// all six node sums, the cross product, the normal and the area.  Persistent over capsules.
+constexpr int GM_THREADS = 256;
+__global__ void __launch_bounds__(GM_THREADS)
+    k_geom(Tables T, long long ncap, const double* __restrict__ x, const double* __restrict__ y,
+           const double* __restrict__ z, double* __restrict__ nx, double* __restrict__ ny,
+           double* __restrict__ nz, double* __restrict__ area) {
+  extern __shared__ double lds[];
+  const int tid = threadIdx.x;
+  const int N = T.N, NN = N * N, np2 = T.np2, nc = N * np2;
+  double* c2 = lds;                 // [np2][N]
+  double* s2 = c2 + np2 * N;        // [np2][N]
+  double* U = s2 + np2 * N;         // [3][N][N][2] longitude DFT
+  double* ca = U + 6 * NN;          // [3][N][N]
+  double* cb = ca + 3 * NN;         // [3][N][N]
+  double* tmp = cb + 3 * NN;        // [6][N][N][2]: x_theta (0..2), x_phi (3..5) partial sums
+  double* pts = tmp;                // [3][nc] the capsule's points (dead before tmp is written)
+  for (int t = tid; t < np2 * N; t += GM_THREADS) c2[t] = T.cs2[t], s2[t] = T.sn2[t];
+  for (long long cap = blockIdx.x; cap < ncap; cap += gridDim.x) {
+    // the capsule's points into LDS with coalesced loads
+    for (int t = tid; t < 3 * nc; t += GM_THREADS) {
+      const int c = t / nc, q = t - c * nc;
+      pts[t] = (c == 0 ? x : (c == 1 ? y : z))[cap * nc + q];
+    }
+    __syncthreads();
+    for (int t = tid; t < 3 * NN; t += GM_THREADS) {
+      const int c = t / NN, i = (t / N) % N, m = t % N;
+      const double* f = pts + c * nc + i * np2;
+      double cs = 0.0, sn = 0.0;
+      #pragma unroll 4
+      for (int j = 0; j < np2; ++j) {
+        const double v = f[j];
+        cs += v * c2[j * N + m];
+        sn += v * s2[j * N + m];
+      }
+      U[2 * t] = cs, U[2 * t + 1] = sn;
+    }
+    __syncthreads();
+    for (int t = tid; t < 3 * NN; t += GM_THREADS) {
+      const int c = t / NN, m = (t / N) % N, n = t % N;
+      double sa = 0.0, sb = 0.0;
+      if (n >= m)
+        #pragma unroll 4
+        for (int i = 0; i < N; ++i) {
+          const double p = T.w2[i] * T.Pc[(i * N + m) * N + n];
+          const int u = (c * N + i) * N + m;
+          sa += p * U[2 * u];
+          sb += p * U[2 * u + 1];
+        }
+      ca[(c * N + m) * N + n] = 2.0 / np2 * sa;
+      cb[(c * N + m) * N + n] = -2.0 / np2 * sb;
+    }
+    __syncthreads();
+    for (int t = tid; t < 6 * NN; t += GM_THREADS) {
+      const int f = t / NN, i = (t / N) % N, m = t % N;
+      const bool dth = f < 3;
+      const int c = dth ? f : f - 3;
+      const double h = m == 0 ? 0.5 : 1.0;
+      const double* P = (dth ? T.Pcd : T.Pc) + (i * N + m) * N;
+      double sa = 0.0, sb = 0.0;
+      #pragma unroll 4
+      for (int n = m; n < N; ++n) {
+        sa += P[n] * ca[(c * N + m) * N + n];
+        sb += P[n] * cb[(c * N + m) * N + n];
+      }
+      tmp[2 * t] = h * sa, tmp[2 * t + 1] = h * sb;
+    }
+    __syncthreads();
+    for (int node = tid; node < nc; node += GM_THREADS) {
+      const int i = node / np2, j = node % np2;
+      double xt[3], xp[3];
+      for (int f = 0; f < 6; ++f) {
+        const double* tf = tmp + 2 * (f * NN + i * N);
+        double v = 0.0;
+        #pragma unroll 4
+        for (int m = 0; m < N; ++m) {
+          const double A = tf[2 * m], B = tf[2 * m + 1];
+          const double cm = c2[j * N + m], sm = s2[j * N + m];
+          if (f < 3)
+            v += A * cm - B * sm;
+          else
+            v += (double)m * (-A * sm - B * cm);
+        }
+        if (f < 3)
+          xt[f] = v;
+        else
+          xp[f - 3] = v;
+      }
+      double cr[3];
+      cross3(xt, xp, cr);
+      const double J = sqrt(dot3(cr, cr));
+      const size_t o = (size_t)cap * nc + node;
+      nx[o] = cr[0] / J, ny[o] = cr[1] / J, nz[o] = cr[2] / J;
+      area[o] = J * T.w2[i] / T.s2[i] * (2.0 * M_PI / np2);
+    }
+    __syncthreads();
+  }
+}
+
+size_t geom_lds_bytes(const Tables& T) {
+  const int N = T.N, NN = N * N;
+  return (size_t)(2 * T.np2 * N + 6 * NN + 6 * NN + 12 * NN) * sizeof(double);
+}
+
 size_t lds_bytes(const Tables& T) {
   const int N = T.N, NN = N * N;
   size_t work = std::max<size_t>(3 * NN * 2, (size_t)RING * 3 * 5 * N * 2 + (size_t)RING * T.np * 6);
@@ -1114,7 +1223,9 @@ int memb_plan_create(memb_plan** out, int L, int device) {
       hipFuncSetAttribute((const void*)k_mem<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)p->lds) != hipSuccess ||
       hipFuncSetAttribute((const void*)k_mem<3>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)p->lds) != hipSuccess)
+                          (int)p->lds) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_geom, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)geom_lds_bytes(p->T)) != hipSuccess)
     return bail(fail(p, MEMB_E_HIP, "hipFuncSetAttribute failed"));
   if ((rc = launch<1>(p, 1, 0.0, 0.0, dsx, dsx + N * np2, dsx + 2 * N * np2, p->ref, nullptr,
                       nullptr, nullptr, nullptr, nullptr, nullptr)))
@@ -1182,7 +1293,13 @@ int memb_geometry_f64_dev(memb_plan* p, int64_t ncap, const double* x, const dou
   if (ncap > 0 && !(x && y && z && nx && ny && nz && area))
     return fail(p, MEMB_E_ARG, "null pointer");
   MCHK(p, hipSetDevice(p->device));
-  return launch<3>(p, ncap, 0.0, 0.0, x, y, z, nullptr, nullptr, nullptr, nx, ny, nz, area);
+  if (ncap == 0) return MEMB_OK;
+  // two 256-thread workgroups per CU (LDS: 2 x 65 KB at L = 16)
+  const unsigned nwg = (unsigned)std::min<long long>(ncap, (long long)p->num_wg / 2);
+  k_geom<<<nwg, GM_THREADS, geom_lds_bytes(p->T), p->stream>>>(p->T, ncap, x, y, z, nx, ny, nz,
+                                                               area);
+  MCHK(p, hipGetLastError());
+  return MEMB_OK;
 }
 
 int memb_traction_select_f64_dev(memb_plan* p, int64_t ncap, const int* sel, double gs,

@@ -90,6 +90,49 @@ def test_membrane_law_matches_oracle(L, eb):
     plan.close()
 
 
+@pytest.mark.parametrize("L", [8, 16, 23])
+def test_geometry_matches_the_laws_areas_and_oracle(L):
+    """memb_geometry_f64_dev (normals and areas only): the areas are the law's own (same
+    formula from the coarse shape; 1e-12 of the largest against the oracle), the normals are
+    unit vectors of the force grid, within the dealiasing's difference of the law's filtered
+    normals, and the same for every capsule of a larger batch (persistent workgroups)."""
+    torch = _torch()
+    from suguru_amd.membrane import MembranePlan
+
+    print(f"case: geometry L={L}", flush=True)
+    ncap = 7
+    x, y, z = capsules(L, ncap, seed=40 + L)
+    plan = MembranePlan(L)
+    plan.set_stream(torch.cuda.current_stream().cuda_stream)
+    n = plan.npts
+    dev = "cuda:0"
+    dx, dy, dz = (torch.from_numpy(a).to(dev) for a in (x, y, z))
+    geo = [torch.full((ncap * n,), np.nan, dtype=torch.float64, device=dev) for _ in range(4)]
+    plan.geometry_dev(dx, dy, dz, *geo)
+    plan.sync()
+    gn = np.stack([g.cpu().numpy() for g in geo[:3]])
+    ga = geo[3].cpu().numpy()
+    o = oracle_lib()
+    for c in range(ncap):
+        sl = slice(c * n, (c + 1) * n)
+        want = [np.zeros(n) for _ in range(7)]
+        xc, yc, zc = (np.ascontiguousarray(a[sl]) for a in (x, y, z))
+        o.mem_traction(L, 6250.0, 1875.0, xc.ctypes.data_as(D), yc.ctypes.data_as(D),
+                       zc.ctypes.data_as(D), *[w.ctypes.data_as(D) for w in want])
+        assert np.abs(ga[sl] - want[6]).max() < 1e-12 * np.abs(want[6]).max(), c
+        assert np.abs(np.linalg.norm(gn[:, sl], axis=0) - 1.0).max() < 1e-13
+        assert np.min((gn[:, sl] * np.stack(want[3:6])).sum(0)) > 0.99
+    # a batch larger than one round of workgroups gives each capsule the same values
+    reps = 3000
+    big = [t.repeat(reps) for t in (dx, dy, dz)]
+    out = [torch.empty(reps * ncap * n, dtype=torch.float64, device=dev) for _ in range(4)]
+    plan.geometry_dev(*big, *out)
+    plan.sync()
+    for g, b in zip(geo, out):
+        assert torch.equal(b.view(reps, -1), g.view(1, -1).expand(reps, -1))
+    plan.close()
+
+
 def test_shape_coefficients_match_oracle():
     torch = _torch()
     from suguru_amd.membrane import MembranePlan

@@ -23,7 +23,8 @@ compares the step time with the learned membrane traction against the plain FP64
 the host cores (a Fortran restatement at -O0 and -O2, serial and OpenMP) on a bounded sample
 of the same workload (`--no-cpu` skips it).  `capsule_step` times the whole capsule step
 without the fluid update (interpolation, FP64 membrane law, spread, position update;
-`--no-capsule-step` skips it).
+`--no-capsule-step` skips it).  `--mlp-only` prints the surrogate network's BF16 inference
+throughput at `--mlp-batch` rows (default 2048) and nothing else.
 """
 from __future__ import annotations
 
@@ -163,6 +164,32 @@ def surrogate_leg():
             "fallback_fraction", "accepted_rel_err_median", "accepted_rel_err_max",
             "t_law_ms", "t_surrogate_ms")
     return {k: r[k] for k in keep}
+
+
+def mlp_inference_leg(torch, batch=2048, reps=50, warmup=5):
+    """Inference throughput of the surrogate network alone (suguru_amd.membrane.SurrogateMLP):
+    3 x 289 SH coefficients -> 4 hidden layers of 512 (SiLU) -> 578 x 3 points, BF16 MFMA with
+    FP32 accumulation, random weights and inputs, `batch` rows per forward pass on the
+    network's own stream.  FLOPs are the unpadded products, 2 x batch x sum(in x out) per pass;
+    the fraction is of MI355X's dense BF16 peak (2.5 PFLOP/s)."""
+    from suguru_amd.surrogate import Surrogate
+
+    sur = Surrogate(L=16, max_batch=batch)
+    sur.mlp.init(3)
+    dims = [sur.din] + [512] * 4 + [sur.dout]
+    flop = 2.0 * batch * sum(a * b for a, b in zip(dims[:-1], dims[1:]))
+    X = torch.randn(batch, sur.din, dtype=torch.float32, device=sur.dev)
+    for _ in range(warmup):
+        sur.predict(X)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        sur.predict(X)
+    torch.cuda.synchronize()
+    ms = 1000.0 * (time.perf_counter() - t0) / reps
+    return {"batch": batch, "layers": dims, "gflop_per_pass": flop / 1e9, "ms_per_pass": ms,
+            "tflops": flop / (ms * 1e-3) / 1e12, "frac_of_bf16_peak": flop / (ms * 1e-3) / 2.5e15,
+            "timing": "wall clock over %d passes between two synchronisations" % reps}
 
 
 def capsule_step_leg(torch, steps=20, warmup=3):
@@ -375,6 +402,9 @@ def main():
                     help="skip the capsule-step leg (interp, membrane law, spread, Update_Pos)")
     ap.add_argument("--no-coupled", action="store_true",
                     help="skip the coupled-step leg (dense 8000-capsule channel with the flow update)")
+    ap.add_argument("--mlp-only", action="store_true",
+                    help="only the surrogate network's inference throughput")
+    ap.add_argument("--mlp-batch", type=int, default=2048, help="rows per pass for --mlp-only")
     ap.add_argument("--coupled-only", action="store_true",
                     help="run the coupled-step leg alone and print it")
     ap.add_argument("--coupled-steps", type=int, default=100,
@@ -400,6 +430,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copied_bytes": a.numel() * 8, "copies": 3}))
+        return
+
+    if args.mlp_only:
+        print(json.dumps({"mlp_inference": mlp_inference_leg(torch, batch=args.mlp_batch)}))
         return
 
     if args.coupled_only:

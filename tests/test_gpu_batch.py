@@ -86,63 +86,65 @@ def test_batch_every_config_graph_and_eager_bitexact(gpu_ctx_factory, moderate):
         t[k] = torch.empty(g.ncells_halo, dtype=torch.float64, device="cuda:0")
     torch.cuda.synchronize()
     b = Batch(ctx, caps.cap_off)
-    b.bind(**t)
-    names = Batch.config_names()
-    want_i = [bits(a) for a in ref_i]
-    want_s = [bits(a.reshape(-1, order="F")) for a in ref_s]
-    for cfg, name in enumerate(names):
+    try:
+        b.bind(**t)
+        names = Batch.config_names()
+        want_i = [bits(a) for a in ref_i]
+        want_s = [bits(a.reshape(-1, order="F")) for a in ref_s]
+        for cfg, name in enumerate(names):
+            for graph in (True, False):
+                print("case: batch config", name, "graph" if graph else "eager", flush=True)
+                b.set_config(cfg)
+                b.capture(use_graph=graph)
+                for rep in range(3):
+                    for k in ("ul", "vl", "wl", "forcex", "forcey", "forcez"):
+                        t[k].fill_(np.nan)  # torch's eager work between replays
+                    torch.cuda.synchronize()
+                    b.run()
+                    ctx.sync()
+                    for k, w in zip(("ul", "vl", "wl"), want_i):
+                        assert np.array_equal(bits(t[k].cpu().numpy()), w), (name, graph, rep, k)
+                    for k, w in zip(("forcex", "forcey", "forcez"), want_s):
+                        assert np.array_equal(bits(t[k].cpu().numpy()), w), (name, graph, rep, k)
+        prof = b.profile(reps=3)
+        assert prof["total_us"] > 0
+        # interpolation on the side stream, concurrent with the spread: same bits
+        b.set_config(0)
+        b.set_overlap(True)
         for graph in (True, False):
-            print("case: batch config", name, "graph" if graph else "eager", flush=True)
-            b.set_config(cfg)
+            print("case: overlap", "graph" if graph else "eager", flush=True)
             b.capture(use_graph=graph)
-            for rep in range(3):
+            for rep in range(2):
                 for k in ("ul", "vl", "wl", "forcex", "forcey", "forcez"):
-                    t[k].fill_(np.nan)  # torch's eager work between replays
+                    t[k].fill_(np.nan)
                 torch.cuda.synchronize()
                 b.run()
                 ctx.sync()
+                torch.cuda.synchronize()
                 for k, w in zip(("ul", "vl", "wl"), want_i):
-                    assert np.array_equal(bits(t[k].cpu().numpy()), w), (name, graph, rep, k)
+                    assert np.array_equal(bits(t[k].cpu().numpy()), w), ("overlap", graph, rep, k)
                 for k, w in zip(("forcex", "forcey", "forcez"), want_s):
-                    assert np.array_equal(bits(t[k].cpu().numpy()), w), (name, graph, rep, k)
-    prof = b.profile(reps=3)
-    assert prof["total_us"] > 0
-    # interpolation on the side stream, concurrent with the spread: same bits
-    b.set_config(0)
-    b.set_overlap(True)
-    for graph in (True, False):
-        print("case: overlap", "graph" if graph else "eager", flush=True)
-        b.capture(use_graph=graph)
-        for rep in range(2):
-            for k in ("ul", "vl", "wl", "forcex", "forcey", "forcez"):
+                    assert np.array_equal(bits(t[k].cpu().numpy()), w), ("overlap", graph, rep, k)
+        b.set_overlap(False)
+        # timed graph instances (the benchmark's timed region): same bits, one timing record each
+        print("case: timed slots", flush=True)
+        b.set_config(0)
+        b.capture_timed(3)
+        b.reset_slots()
+        for rep in range(3):
+            for k in ("forcex", "forcey", "forcez"):
                 t[k].fill_(np.nan)
             torch.cuda.synchronize()
             b.run()
             ctx.sync()
-            torch.cuda.synchronize()
-            for k, w in zip(("ul", "vl", "wl"), want_i):
-                assert np.array_equal(bits(t[k].cpu().numpy()), w), ("overlap", graph, rep, k)
             for k, w in zip(("forcex", "forcey", "forcez"), want_s):
-                assert np.array_equal(bits(t[k].cpu().numpy()), w), ("overlap", graph, rep, k)
-    b.set_overlap(False)
-    # timed graph instances (the benchmark's timed region): same bits, one timing record each
-    print("case: timed slots", flush=True)
-    b.set_config(0)
-    b.capture_timed(3)
-    b.reset_slots()
-    for rep in range(3):
-        for k in ("forcex", "forcey", "forcez"):
-            t[k].fill_(np.nan)
-        torch.cuda.synchronize()
-        b.run()
-        ctx.sync()
-        for k, w in zip(("forcex", "forcey", "forcez"), want_s):
-            assert np.array_equal(bits(t[k].cpu().numpy()), w), ("timed", rep, k)
-    times = b.slot_times()
-    assert len(times) == 3
-    for s in times:
-        assert 0 < s["tile_spread_us"] < s["step_us"] and s["interp_us"] > 0
-    b.close()
+                assert np.array_equal(bits(t[k].cpu().numpy()), w), ("timed", rep, k)
+        times = b.slot_times()
+        assert len(times) == 3
+        for s in times:
+            assert 0 < s["tile_spread_us"] < s["step_us"] and s["interp_us"] > 0
+    finally:
+        b.close()
 
 
 def test_batch_replays_follow_moving_points(gpu_ctx_factory):
@@ -166,18 +168,20 @@ def test_batch_replays_follow_moving_points(gpu_ctx_factory):
     um = torch.empty((caps.ncap, 3), dtype=torch.float64, device="cuda:0")
     torch.cuda.synchronize()
     b = Batch(ctx, caps.cap_off)
-    b.bind(**t)
-    b.capture(use_graph=True)
-    x, y, z = caps.x.copy(), caps.y.copy(), caps.z.copy()
-    for step in range(4):
-        b.run()
-        ctx.update_pos_dev(off, t["x"], t["y"], t["z"], t["ul"], t["vl"], t["wl"], 2e-3, cen, um)
-        ctx.sync()
-        ri = O.eulr2lagr(g, wl.u, wl.v, wl.w, caps.cap_off, x, y, z)
-        O.update_pos(g, caps.cap_off, 2e-3, x, y, z, *[np.array(a) for a in ri])
-        for k, a in zip("xyz", (x, y, z)):
-            assert np.array_equal(bits(t[k].cpu().numpy()), bits(a)), (step, k)
-    b.close()
+    try:
+        b.bind(**t)
+        b.capture(use_graph=True)
+        x, y, z = caps.x.copy(), caps.y.copy(), caps.z.copy()
+        for step in range(4):
+            b.run()
+            ctx.update_pos_dev(off, t["x"], t["y"], t["z"], t["ul"], t["vl"], t["wl"], 2e-3, cen, um)
+            ctx.sync()
+            ri = O.eulr2lagr(g, wl.u, wl.v, wl.w, caps.cap_off, x, y, z)
+            O.update_pos(g, caps.cap_off, 2e-3, x, y, z, *[np.array(a) for a in ri])
+            for k, a in zip("xyz", (x, y, z)):
+                assert np.array_equal(bits(t[k].cpu().numpy()), bits(a)), (step, k)
+    finally:
+        b.close()
 
 
 def test_nonfinite_forces_rejected_every_config(gpu_ctx_factory):

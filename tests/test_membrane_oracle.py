@@ -4,7 +4,9 @@ The reference's law (forcing.f90:11-114, sph.f90:1523-1755) needs GSL, which is 
 cannot be compiled here; its neo-Hookean coefficients (membmodel.c) can, and are compared
 directly.  Everything else is pinned by what the mathematics fixes:
 
-* spherical-harmonic machinery: Gauss-Legendre nodes/weights against numpy, Legendre
+* spherical-harmonic machinery: Gauss-Legendre nodes/weights against numpy and the colatitudes
+  against SPHEREPACK's own test log (gaqd at nlat = 15 and 33, as its tdiv and tgrad tests
+  print them), Legendre
   normalisation against scipy (without the Condon-Shortley phase), theta-derivatives against
   finite differences, and exact analysis/synthesis round trips;
 * the law: zero traction on the reference sphere; Laplace-law pressure 2 tau / lambda with the
@@ -98,6 +100,31 @@ def test_gauss_nodes_match_numpy(lib):
         for k in range(0, 2 * n):
             exact = 2.0 / (k + 1) if k % 2 == 0 else 0.0
             assert abs((w * xc ** k).sum() - exact) < 1e-15, (n, k)
+
+
+# SPHEREPACK's own test log: the Gauss colatitudes gaqd returns, as tdiv (nlat = 15) and
+# tgrad (nlat = 33) print them in E11.4 (spherepack3.2_impi-15/output/darwin.dp:250-252 and
+# :448-453)
+SPHEREPACK_THTG = {
+    15: "0.1551E+00 0.3561E+00 0.5582E+00 0.7606E+00 0.9631E+00 0.1166E+01 0.1368E+01 "
+        "0.1571E+01 0.1773E+01 0.1976E+01 0.2178E+01 0.2381E+01 0.2583E+01 0.2786E+01 "
+        "0.2986E+01",
+    33: "0.7178E-01 0.1648E+00 0.2583E+00 0.3520E+00 0.4457E+00 0.5394E+00 0.6332E+00 "
+        "0.7269E+00 0.8207E+00 0.9144E+00 0.1008E+01 0.1102E+01 0.1196E+01 0.1289E+01 "
+        "0.1383E+01 0.1477E+01 0.1571E+01 0.1665E+01 0.1758E+01 0.1852E+01 0.1946E+01 "
+        "0.2040E+01 0.2133E+01 0.2227E+01 0.2321E+01 0.2415E+01 0.2508E+01 0.2602E+01 "
+        "0.2696E+01 0.2790E+01 0.2883E+01 0.2977E+01 0.3070E+01",
+}
+
+
+@pytest.mark.parametrize("n", sorted(SPHEREPACK_THTG))
+def test_gauss_colatitudes_match_spherepack_log(lib, n):
+    from suguru_amd.formats import fortran_e
+
+    th, w = np.zeros(n), np.zeros(n)
+    lib.mem_gauss(n, ptr(th), ptr(w))
+    got = [fortran_e(t, 11, 4).strip() for t in th]
+    assert got == SPHEREPACK_THTG[n].split()
 
 
 def test_legendre_normalisation_matches_scipy(lib):

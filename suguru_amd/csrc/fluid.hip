@@ -7,8 +7,9 @@
 //   1. x transform of every (j, k) line (k_rfft, lines contiguous),
 //   2. y transform of every (i, k) line (k_rfft, lines = 16 neighbouring columns, so that each
 //      row segment is one 128-byte load),
-//   3. Gaussian elimination in k per (i, j) mode (k_tri: one lane per mode, the forward sweep
-//      kept in LDS, the per-mode reciprocal pivots read from a table built once per context),
+//   3. Gaussian elimination in k per (i, j) mode (k_tri_seg: one lane per mode, the k rows in
+//      register-held segments joined through LDS, the per-mode reciprocal pivots read from a
+//      table built once per context; k_tri, two passes through the array, above 257 planes),
 //   4. y inverse, 5. x inverse (into the caller's field interior).
 // Transforms.  A real line of length N = 2M is transformed as M complex values
 // z_n = x_2n + i x_2n+1 (radix-2, in LDS, input bit-reversed on load), then split into the
@@ -267,6 +268,119 @@ __global__ void __launch_bounds__(TRI_MODES)
     const double v = LDS ? fw[k * TRI_MODES + threadIdx.x] : p[k * plane + m];
     nxt = v - d * nxt;
     p[k * plane + m] = nxt;
+  }
+}
+
+// One pass over the array for the same elimination.  A workgroup takes TRI_MODES modes (the
+// lanes) and splits rows 0..ktot-2 into segments of TRS_LSEG rows, one wavefront each, held in
+// registers with their reciprocal pivots.  Each recurrence of the sweep is affine in the value
+// it carries in from the previous segment, so
+//   forward  y_k = (r_k - a_k y_k-1) z_k      over a segment: y_end = y0_end + G y_in,
+//   backward x_k = y_k - d_k x_k+1             over a segment: x_start = x0_start + H x_in,
+// where (y0, G) and (x0, H) come from a first pass started at 0 (G, H the products of the
+// coefficients -a z and -d).  The carried values are composed through LDS, then every segment
+// runs the sweep again from its true incoming value (the same expressions as k_tri inside a
+// segment; only the carried values are re-associated).  The last wavefront also solves the
+// last row (solver.f90:103-107, with its zero-pivot guard).  Reads the array and the pivots
+// once and writes the solution once, against twice each for k_tri.
+constexpr int TRS_LSEG = 16;                   // rows per segment (registers per lane: 2 x 16)
+constexpr int TRS_MAXSEG = 16;                 // segments (wavefronts) per workgroup
+__global__ void __launch_bounds__(TRI_MODES * TRS_MAXSEG)
+    k_tri_seg(double* __restrict__ p, int itot, int jtot, int ktot, const double* __restrict__ a,
+              const double* __restrict__ c, const double* __restrict__ ztab) {
+  __shared__ double fy0[TRS_MAXSEG][TRI_MODES], fg[TRS_MAXSEG][TRI_MODES];
+  __shared__ double bx0[TRS_MAXSEG][TRI_MODES], bh[TRS_MAXSEG][TRI_MODES];
+  __shared__ double xlast[TRI_MODES];
+  const int lane = threadIdx.x & (TRI_MODES - 1);
+  // the segment is uniform over the wavefront: its rows and coefficients live in scalars
+  const int s = __builtin_amdgcn_readfirstlane(threadIdx.x / TRI_MODES);
+  const int nseg = blockDim.x / TRI_MODES;
+  const long long plane = (long long)itot * jtot;
+  const long long m = (long long)blockIdx.x * TRI_MODES + lane;
+  const bool live = m < plane;
+  const int nk = ktot - 1;  // rows of the two recurrences
+  const int k0 = min(s * TRS_LSEG, nk), n = min(TRS_LSEG, nk - k0);
+  long long zb = 0;
+  if (live) {
+    const int i = (int)(m % itot), j = (int)(m / itot);
+    zb = (long long)wavenumber(j, jtot) * (itot / 2 + 1) + wavenumber(i, itot);
+  }
+  const long long zs = (long long)(itot / 2 + 1) * (jtot / 2 + 1);
+  // rows t >= n of a short segment are padding: they leave every carried value unchanged
+  // (selects, so that the arrays stay in registers)
+  // (every load is in bounds: lanes past the last mode read mode 0, then discard it)
+  const long long mm = live ? m : 0;
+  double r[TRS_LSEG], z[TRS_LSEG];
+#pragma unroll
+  for (int t = 0; t < TRS_LSEG; ++t) {
+    const int k = k0 + min(t, max(n - 1, 0));
+    const double rv = p[k * plane + mm], zv = ztab[k * zs + zb];
+    r[t] = live && n > 0 ? rv : 0.0;
+    z[t] = live && n > 0 ? zv : 1.0;
+  }
+  // forward, first pass: the segment's map y_in -> y_end = y0 + G y_in
+  {
+    double y = 0.0, gm = 1.0;
+#pragma unroll
+    for (int t = 0; t < TRS_LSEG; ++t) {
+      const double ak = a[k0 + min(t, max(n - 1, 0))];
+      const double yn = (r[t] - ak * y) * z[t];
+      const double gn = gm * (-(ak * z[t]));
+      y = t < n ? yn : y;
+      gm = t < n ? gn : gm;
+    }
+    fy0[s][lane] = y, fg[s][lane] = gm;
+  }
+  __syncthreads();
+  double yin = 0.0;
+  for (int q = 0; q < s; ++q) yin = fy0[q][lane] + fg[q][lane] * yin;
+  // forward, second pass from the true incoming value (r becomes y)
+  double prev = yin;
+#pragma unroll
+  for (int t = 0; t < TRS_LSEG; ++t) {
+    const double yn = (r[t] - a[k0 + min(t, max(n - 1, 0))] * prev) * z[t];
+    prev = t < n ? yn : prev;
+    r[t] = prev;
+  }
+  // the last row, by the last wavefront: y_{ktot-2} is its carried-out value
+  if (s == nseg - 1) {
+    double nxt = 0.0;
+    if (live) {
+      const double zl = ztab[(long long)(ktot - 1) * zs + zb];
+      nxt = zl != 0.0 ? (p[(long long)(ktot - 1) * plane + m] - a[ktot - 1] * prev) / zl : 0.0;
+      p[(long long)(ktot - 1) * plane + m] = nxt;
+    }
+    xlast[lane] = nxt;
+  }
+  // backward, first pass: the segment's map x_in -> x_start = x0 + H x_in, d_k = c_k z_k
+  {
+    double x = 0.0, hm = 1.0;
+#pragma unroll
+    for (int t = TRS_LSEG - 1; t >= 0; --t) {
+      const double d = c[k0 + min(t, max(n - 1, 0))] * z[t];
+      const double xn = r[t] - d * x;
+      const double hn = hm * (-d);
+      x = t < n ? xn : x;
+      hm = t < n ? hn : hm;
+    }
+    bx0[s][lane] = x, bh[s][lane] = hm;
+  }
+  __syncthreads();
+  double xin = xlast[lane];
+  for (int q = nseg - 1; q > s; --q) xin = bx0[q][lane] + bh[q][lane] * xin;
+  // backward, second pass: the solution
+  double nxt = xin;
+#pragma unroll
+  for (int t = TRS_LSEG - 1; t >= 0; --t) {
+    const double d = c[k0 + min(t, max(n - 1, 0))] * z[t];
+    const double xn = r[t] - d * nxt;
+    nxt = t < n ? xn : nxt;
+    r[t] = nxt;
+  }
+  if (live) {
+#pragma unroll
+    for (int t = 0; t < TRS_LSEG; ++t)
+      if (t < n) p[(k0 + t) * plane + m] = r[t];
   }
 }
 
@@ -828,7 +942,11 @@ int tri(fld_ctx* c, double* a, int ne) {
   const long long modes = (long long)g.itot * g.jtot;
   const double* ac = c->coef[ne];
   const double* cc = c->coef[ne] + g.ktot;
-  if (c->tri_lds)
+  const int nseg = std::max(1, (g.ktot - 1 + TRS_LSEG - 1) / TRS_LSEG);
+  if (nseg <= TRS_MAXSEG)
+    k_tri_seg<<<nblocks(modes, TRI_MODES), TRI_MODES * nseg, 0, c->stream>>>(
+        a, g.itot, g.jtot, g.ktot, ac, cc, c->ztab[ne]);
+  else if (c->tri_lds)
     k_tri<true><<<nblocks(modes, TRI_MODES), TRI_MODES, (size_t)TRI_MODES * g.ktot * sizeof(double),
                   c->stream>>>(a, g.itot, g.jtot, g.ktot, ac, cc, c->ztab[ne]);
   else

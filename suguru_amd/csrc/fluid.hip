@@ -5,8 +5,8 @@
 // in compact spectral arrays of itot x jtot x ktot doubles (i fastest), one per velocity
 // component; the pressure solve reuses the first.  A solve is five passes:
 //   1. x transform of every (j, k) line (k_rfft, lines contiguous),
-//   2. y transform of every (i, k) line (k_rfft, lines = 16 neighbouring columns, so that each
-//      row segment is one 128-byte load),
+//   2. y transform of every (i, k) line (k_rfft, lines = 4 (forward) or 8 (inverse)
+//      neighbouring columns per workgroup: row segments of 32 or 64 bytes),
 //   3. Gaussian elimination in k per (i, j) mode (k_tri_seg: one lane per mode, the k rows in
 //      register-held segments joined through LDS, the per-mode reciprocal pivots read from a
 //      table built once per context; k_tri, two passes through the array, above 257 planes),
@@ -33,7 +33,8 @@ namespace {
 
 constexpr int FFT_THREADS = 256;
 constexpr int TRI_MODES = 64;        // modes (lanes) per elimination workgroup
-constexpr int Y_LINES = 8;           // columns per y-transform workgroup
+constexpr int Y_LINES_FWD = 4;       // columns per forward y-transform workgroup
+constexpr int Y_LINES_INV = 8;       // columns per inverse y-transform workgroup
 constexpr size_t LDS_MAX = 160 * 1024;
 
 struct FGrid {
@@ -811,8 +812,8 @@ struct fld_ctx {
   double* part = nullptr;  // chkdiv partials
   int logmx = 0, logmy = 0;
   bool tri_lds = true;
-  size_t lds_y = 0;
-  int ylines = Y_LINES, xlines = 1;
+  size_t lds_yf = 0, lds_yi = 0;
+  int ylines_f = Y_LINES_FWD, ylines_i = Y_LINES_INV, xlines = 1;
 };
 
 namespace {
@@ -923,15 +924,15 @@ int fft_x(fld_ctx* c, bool inv, const double* src, bool src_field, double* dst, 
 // y transform in place on a compact array
 int fft_y(fld_ctx* c, bool inv, double* a) {
   const FGrid& g = c->g;
-  const int N = g.jtot, nl = c->ylines;
+  const int N = g.jtot, nl = inv ? c->ylines_i : c->ylines_f;
   const int gpp = g.itot / nl;
   const long long cp = (long long)g.itot * g.jtot;
   const unsigned nb = (unsigned)(gpp * g.ktot);
   if (!inv)
-    k_rfft_fwd<false><<<nb, FFT_THREADS, c->lds_y, c->stream>>>(
+    k_rfft_fwd<false><<<nb, FFT_THREADS, c->lds_yf, c->stream>>>(
         a, a, N, c->logmy, nl, gpp, cp, cp, nl, nl, g.itot, 1, g.itot, 1, 0, c->tw_y);
   else
-    k_rfft_inv<false><<<nb, FFT_THREADS, c->lds_y, c->stream>>>(
+    k_rfft_inv<false><<<nb, FFT_THREADS, c->lds_yi, c->stream>>>(
         a, a, N, c->logmy, nl, gpp, cp, cp, nl, nl, g.itot, 1, g.itot, 1, 0, c->tw_y);
   HIPF(c, hipGetLastError());
   return FLD_OK;
@@ -997,10 +998,16 @@ int fld_create(fld_ctx** out, const ibx_grid* gr, double rep, double dt, int dev
   c->ncomp = (long long)g.itot * g.jtot * g.ktot;
   c->logmx = ilog2(g.itot / 2), c->logmy = ilog2(g.jtot / 2);
   c->xlines = std::min(g.jtot, std::max(1, 2048 / g.itot));
-  c->ylines = std::min(g.itot, Y_LINES);
-  while (c->ylines > 1 && (size_t)c->ylines * (g.jtot / 2 + 1) * sizeof(double2) > LDS_MAX / 2)
-    c->ylines >>= 1;
-  c->lds_y = (size_t)c->ylines * (g.jtot / 2 + 1) * sizeof(double2);
+  // the forward y transform runs more workgroups of fewer columns (4: 760 us against 881 us
+  // with 8 at 1024 x 512 x 256); the inverse is fastest with 8
+  auto lines = [&](int want) {
+    int nl = std::min(g.itot, want);
+    while (nl > 1 && (size_t)nl * (g.jtot / 2 + 1) * sizeof(double2) > LDS_MAX / 2) nl >>= 1;
+    return nl;
+  };
+  c->ylines_f = lines(Y_LINES_FWD), c->ylines_i = lines(Y_LINES_INV);
+  c->lds_yf = (size_t)c->ylines_f * (g.jtot / 2 + 1) * sizeof(double2);
+  c->lds_yi = (size_t)c->ylines_i * (g.jtot / 2 + 1) * sizeof(double2);
   // the sweep's forward values in LDS limit a compute unit to one 64-lane workgroup at
   // ktot = 256; through the spectral array itself every compute unit runs many wavefronts
   // and the extra pass is cheaper than the idle lanes

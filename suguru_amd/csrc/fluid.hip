@@ -32,6 +32,10 @@
 namespace {
 
 constexpr int FFT_THREADS = 256;
+// loads each thread keeps in flight while a transform reads its lines (8 for the forward
+// transforms; 4 for the inverse ones, whose four loads per value would cost occupancy at 8)
+constexpr int FFT_LB_FWD = 8;
+constexpr int FFT_LB_INV = 4;
 constexpr int TRI_MODES = 64;        // modes (lanes) per elimination workgroup
 constexpr int Y_LINES_FWD = 4;       // columns per forward y-transform workgroup
 constexpr int Y_LINES_INV = 8;       // columns per inverse y-transform workgroup
@@ -140,9 +144,22 @@ __global__ void __launch_bounds__(FFT_THREADS)
   const long long bs = off_src + kp * plane_src + gp * gstride_src;
   const long long bd = kp * plane_dst + gp * gstride_dst;
   double* smd = reinterpret_cast<double*>(sm);
-  for (int q = threadIdx.x; q < nl * N; q += blockDim.x) {
-    const int l = XMAJOR ? q >> logn : q & (nl - 1), e = XMAJOR ? q & (N - 1) : q >> lognl;
-    smd[2 * (l * (M + 1) + brev(e >> 1, logm)) + (e & 1)] = src[bs + l * ls_src + e * es_src];
+  // every load of a batch in flight before the LDS stores
+  for (int q0 = threadIdx.x; q0 < nl * N; q0 += blockDim.x * FFT_LB_FWD) {
+    double v[FFT_LB_FWD];
+#pragma unroll
+    for (int u = 0; u < FFT_LB_FWD; ++u) {
+      const int q = q0 + u * blockDim.x;
+      const int l = XMAJOR ? q >> logn : q & (nl - 1), e = XMAJOR ? q & (N - 1) : q >> lognl;
+      v[u] = 0.0;
+      if (q < nl * N) v[u] = src[bs + l * ls_src + e * es_src];
+    }
+#pragma unroll
+    for (int u = 0; u < FFT_LB_FWD; ++u) {
+      const int q = q0 + u * blockDim.x;
+      const int l = XMAJOR ? q >> logn : q & (nl - 1), e = XMAJOR ? q & (N - 1) : q >> lognl;
+      if (q < nl * N) smd[2 * (l * (M + 1) + brev(e >> 1, logm)) + (e & 1)] = v[u];
+    }
   }
   fft_stages<false>(sm, nl, M, logm, logn, tw);
   // X_k = E_k + W^k O_k, E_k = (Z_k + conj Z_M-k) / 2, O_k = (Z_k - conj Z_M-k) / 2i
@@ -180,23 +197,42 @@ __global__ void __launch_bounds__(FFT_THREADS)
   const long long bs = kp * plane_src + gp * gstride_src;
   const long long bd = off_dst + kp * plane_dst + gp * gstride_dst;
   // Z_k = E_k + i O_k, E_k = (X_k + conj X_M-k) / 2, O_k = (X_k - conj X_M-k) conj(W^k) / 2
-  for (int q = threadIdx.x; q < nl * M; q += blockDim.x) {
-    const int l = XMAJOR ? q >> logm : q & (nl - 1), k = XMAJOR ? q & (M - 1) : q >> lognl;
-    const double* in = src + bs + l * ls_src;
-    double2 Z;
-    if (k == 0) {
-      const double x0 = in[0], xm = in[es_src];
-      Z = make_double2(0.5 * (x0 + xm), 0.5 * (x0 - xm));
-    } else {
-      const double2 a = make_double2(in[(2 * k) * es_src], in[(2 * k + 1) * es_src]);
-      const double2 b = make_double2(in[(2 * (M - k)) * es_src], in[(2 * (M - k) + 1) * es_src]);
-      const double2 E = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
-      const double2 D = make_double2(0.5 * (a.x - b.x), 0.5 * (a.y + b.y));
-      const double2 wc = make_double2(tw[k].x, -tw[k].y);
-      const double2 O = cmul(D, wc);
-      Z = make_double2(E.x - O.y, E.y + O.x);
+  for (int q0 = threadIdx.x; q0 < nl * M; q0 += blockDim.x * FFT_LB_INV) {
+    double2 va[FFT_LB_INV], vb[FFT_LB_INV];
+#pragma unroll
+    for (int u = 0; u < FFT_LB_INV; ++u) {
+      const int q = q0 + u * blockDim.x;
+      const int l = XMAJOR ? q >> logm : q & (nl - 1), k = XMAJOR ? q & (M - 1) : q >> lognl;
+      const double* in = src + bs + l * ls_src;
+      va[u] = make_double2(0.0, 0.0), vb[u] = va[u];
+      if (q < nl * M) {
+        if (k == 0) {
+          va[u] = make_double2(in[0], in[es_src]);
+        } else {
+          va[u] = make_double2(in[(2 * k) * es_src], in[(2 * k + 1) * es_src]);
+          vb[u] = make_double2(in[(2 * (M - k)) * es_src], in[(2 * (M - k) + 1) * es_src]);
+        }
+      }
     }
-    sm[l * (M + 1) + brev(k, logm)] = Z;
+#pragma unroll
+    for (int u = 0; u < FFT_LB_INV; ++u) {
+      const int q = q0 + u * blockDim.x;
+      if (q >= nl * M) continue;
+      const int l = XMAJOR ? q >> logm : q & (nl - 1), k = XMAJOR ? q & (M - 1) : q >> lognl;
+      double2 Z;
+      if (k == 0) {
+        const double x0 = va[u].x, xm = va[u].y;
+        Z = make_double2(0.5 * (x0 + xm), 0.5 * (x0 - xm));
+      } else {
+        const double2 a = va[u], b = vb[u];
+        const double2 E = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+        const double2 D = make_double2(0.5 * (a.x - b.x), 0.5 * (a.y + b.y));
+        const double2 wc = make_double2(tw[k].x, -tw[k].y);
+        const double2 O = cmul(D, wc);
+        Z = make_double2(E.x - O.y, E.y + O.x);
+      }
+      sm[l * (M + 1) + brev(k, logm)] = Z;
+    }
   }
   fft_stages<true>(sm, nl, M, logm, logn, tw);
   const double sc = 1.0 / (double)M;

@@ -697,29 +697,48 @@ __device__ __forceinline__ double kstep(const FGrid& g, const double* f, int kin
 
 // bounduvw / boundp in one launch: the k step, then updthalos in x, then in y
 // (bound.f90:61-76, 100-104; bound.f90:141-190).  Every halo or ghost cell takes its final
-// value straight from the interior column the sequence copies it from.  One thread per
-// (column, field); columns are the (i1+1)(j1+1) of the plane.
+// value straight from the interior column the sequence copies it from, and those interior
+// values are not written here, so every cell is independent.  Threads [0, nfields * itot *
+// jtot) take an interior column (its ghost planes k = 0 and ktot+1, and k = ktot for w); the
+// rest one cell of a halo column: x halos i = 0, itot+1 for j = 0..jtot+1, then y halos
+// j = 0, jtot+1 for i = 1..itot, every k (halo index fastest).
 __global__ void __launch_bounds__(256) k_bound(FGrid g, double* __restrict__ f0,
                                                double* __restrict__ f1, double* __restrict__ f2,
                                                int nfields, int kind0) {
-  const long long ncol = g.sy * (long long)(g.jtot + 2);
+  const long long nin = (long long)g.itot * g.jtot;
+  const long long nxh = 2LL * (g.jtot + 2), nh = nxh + 2LL * g.itot, K = g.ktot + 2;
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ncol * nfields) return;
-  const int fi = (int)(t / ncol);
-  const long long col = t - fi * ncol;
-  const int i = (int)(col % g.sy), j = (int)(col / g.sy);
+  if (t < nin * nfields) {
+    const int fi = (int)(t / nin);
+    const long long r = t - fi * nin;
+    const int i = 1 + (int)(r % g.itot), j = 1 + (int)(r / g.itot);
+    double* f = fi == 0 ? f0 : (fi == 1 ? f1 : f2);
+    const int kind = kind0 == 2 ? 2 : (fi == 2 ? 1 : 0);
+    f[fidx(g, i, j, 0)] = kstep(g, f, kind, i, j, 0);
+    f[fidx(g, i, j, g.ktot + 1)] = kstep(g, f, kind, i, j, g.ktot + 1);
+    if (kind == 1) f[fidx(g, i, j, g.ktot)] = 0.0;
+    return;
+  }
+  const long long e = t - nin * nfields;
+  if (e >= nh * K * nfields) return;
+  const int fi = (int)(e / (nh * K));
+  const long long r = e - fi * nh * K;
+  const int k = (int)(r / nh);
+  const long long h = r - k * nh;
+  int i, j;
+  if (h < nxh) {
+    i = h < g.jtot + 2 ? 0 : g.itot + 1;
+    j = (int)(h % (g.jtot + 2));
+  } else {
+    const long long hy = h - nxh;
+    j = hy < g.itot ? 0 : g.jtot + 1;
+    i = 1 + (int)(hy % g.itot);
+  }
   double* f = fi == 0 ? f0 : (fi == 1 ? f1 : f2);
   const int kind = kind0 == 2 ? 2 : (fi == 2 ? 1 : 0);
   const int si = i == 0 ? g.itot : (i == g.itot + 1 ? 1 : i);
   const int sj = j == 0 ? g.jtot : (j == g.jtot + 1 ? 1 : j);
-  const bool inner = si == i && sj == j;
-  if (inner) {
-    f[fidx(g, i, j, 0)] = kstep(g, f, kind, i, j, 0);
-    f[fidx(g, i, j, g.ktot + 1)] = kstep(g, f, kind, i, j, g.ktot + 1);
-    if (kind == 1) f[fidx(g, i, j, g.ktot)] = 0.0;
-  } else {
-    for (int k = 0; k <= g.ktot + 1; ++k) f[fidx(g, i, j, k)] = kstep(g, f, kind, si, sj, k);
-  }
+  f[fidx(g, i, j, k)] = kstep(g, f, kind, si, sj, k);
 }
 
 // indicator.f90:51-67: the central divergence of G = (gx, gy, gz), ((g(+1) - g(-1)) .5) d?i
@@ -1003,8 +1022,10 @@ int solve_compact(fld_ctx* c, double* a, int ne, double* out) {
 }
 
 int bound(fld_ctx* c, double* f0, double* f1, double* f2, int nfields, int kind0) {
-  const long long ncol = c->g.sy * (long long)(c->g.jtot + 2);
-  k_bound<<<nblocks(ncol * nfields, 256), 256, 0, c->stream>>>(c->g, f0, f1, f2, nfields, kind0);
+  const FGrid& g = c->g;
+  const long long n = ((long long)g.itot * g.jtot +
+                       (2LL * (g.jtot + 2) + 2LL * g.itot) * (g.ktot + 2)) * nfields;
+  k_bound<<<nblocks(n, 256), 256, 0, c->stream>>>(g, f0, f1, f2, nfields, kind0);
   HIPF(c, hipGetLastError());
   return FLD_OK;
 }

@@ -144,6 +144,15 @@ __global__ void __launch_bounds__(FFT_THREADS)
   const long long bs = off_src + kp * plane_src + gp * gstride_src;
   const long long bd = kp * plane_dst + gp * gstride_dst;
   double* smd = reinterpret_cast<double*>(sm);
+  // the y transforms read their twiddles from an LDS copy after the lines (each radix-4
+  // butterfly needs three per pass; from the caches they put a round trip in every pass);
+  // the x transforms, whose forward pass was slower that way, read them from memory
+  const double2* twl = tw;
+  if (!XMAJOR) {
+    double2* stw = sm + nl * (M + 1);
+    for (int q = threadIdx.x; q < M; q += blockDim.x) stw[q] = tw[q];
+    twl = stw;
+  }
   // every load of a batch in flight before the LDS stores
   for (int q0 = threadIdx.x; q0 < nl * N; q0 += blockDim.x * FFT_LB_FWD) {
     double v[FFT_LB_FWD];
@@ -161,7 +170,7 @@ __global__ void __launch_bounds__(FFT_THREADS)
       if (q < nl * N) smd[2 * (l * (M + 1) + brev(e >> 1, logm)) + (e & 1)] = v[u];
     }
   }
-  fft_stages<false>(sm, nl, M, logm, logn, tw);
+  fft_stages<false>(sm, nl, M, logm, logn, twl);
   // X_k = E_k + W^k O_k, E_k = (Z_k + conj Z_M-k) / 2, O_k = (Z_k - conj Z_M-k) / 2i
   for (int q = threadIdx.x; q < nl * M; q += blockDim.x) {
     const int l = XMAJOR ? q >> logm : q & (nl - 1), k = XMAJOR ? q & (M - 1) : q >> lognl;
@@ -175,7 +184,7 @@ __global__ void __launch_bounds__(FFT_THREADS)
       const double2 a = z[k], b = z[M - k];
       const double2 E = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
       const double2 O = make_double2(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));
-      const double2 X = cmul(tw[k], O);
+      const double2 X = cmul(twl[k], O);
       out[(2 * k) * es_dst] = E.x + X.x;
       out[(2 * k + 1) * es_dst] = E.y + X.y;
     }
@@ -196,6 +205,10 @@ __global__ void __launch_bounds__(FFT_THREADS)
   const int kp = blk / groups_per_plane, gp = blk - kp * groups_per_plane;
   const long long bs = kp * plane_src + gp * gstride_src;
   const long long bd = off_dst + kp * plane_dst + gp * gstride_dst;
+  // twiddles from an LDS copy after the lines (as in k_rfft_fwd)
+  double2* stw = sm + nl * (M + 1);
+  for (int q = threadIdx.x; q < M; q += blockDim.x) stw[q] = tw[q];
+  __syncthreads();
   // Z_k = E_k + i O_k, E_k = (X_k + conj X_M-k) / 2, O_k = (X_k - conj X_M-k) conj(W^k) / 2
   for (int q0 = threadIdx.x; q0 < nl * M; q0 += blockDim.x * FFT_LB_INV) {
     double2 va[FFT_LB_INV], vb[FFT_LB_INV];
@@ -227,14 +240,14 @@ __global__ void __launch_bounds__(FFT_THREADS)
         const double2 a = va[u], b = vb[u];
         const double2 E = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
         const double2 D = make_double2(0.5 * (a.x - b.x), 0.5 * (a.y + b.y));
-        const double2 wc = make_double2(tw[k].x, -tw[k].y);
+        const double2 wc = make_double2(stw[k].x, -stw[k].y);
         const double2 O = cmul(D, wc);
         Z = make_double2(E.x - O.y, E.y + O.x);
       }
       sm[l * (M + 1) + brev(k, logm)] = Z;
     }
   }
-  fft_stages<true>(sm, nl, M, logm, logn, tw);
+  fft_stages<true>(sm, nl, M, logm, logn, stw);
   const double sc = 1.0 / (double)M;
   const double* smd = reinterpret_cast<const double*>(sm);
   for (int q = threadIdx.x; q < nl * N; q += blockDim.x) {
@@ -962,7 +975,8 @@ int fft_x(fld_ctx* c, bool inv, const double* src, bool src_field, double* dst, 
   const long long row_s = src_field ? g.sy : g.itot, plane_s = src_field ? g.sz : cp;
   const long long row_d = dst_field ? g.sy : g.itot, plane_d = dst_field ? g.sz : cp;
   const long long off = 1 + g.sy + g.sz;  // (1, 1, 1)
-  const size_t lds = (size_t)nl * (N / 2 + 1) * sizeof(double2);
+  // lines, plus the inverse's LDS copy of the twiddles
+  const size_t lds = (size_t)nl * (N / 2 + 1) * sizeof(double2) + (inv ? (size_t)(N / 2) * sizeof(double2) : 0);
   const unsigned nb = (unsigned)(gpp * g.ktot);
   if (!inv)
     k_rfft_fwd<true><<<nb, FFT_THREADS, lds, c->stream>>>(
@@ -1063,8 +1077,9 @@ int fld_create(fld_ctx** out, const ibx_grid* gr, double rep, double dt, int dev
     return nl;
   };
   c->ylines_f = lines(Y_LINES_FWD), c->ylines_i = lines(Y_LINES_INV);
-  c->lds_yf = (size_t)c->ylines_f * (g.jtot / 2 + 1) * sizeof(double2);
-  c->lds_yi = (size_t)c->ylines_i * (g.jtot / 2 + 1) * sizeof(double2);
+  // lines plus the LDS copy of the twiddles
+  c->lds_yf = ((size_t)c->ylines_f * (g.jtot / 2 + 1) + g.jtot / 2) * sizeof(double2);
+  c->lds_yi = ((size_t)c->ylines_i * (g.jtot / 2 + 1) + g.jtot / 2) * sizeof(double2);
   // the sweep's forward values in LDS limit a compute unit to one 64-lane workgroup at
   // ktot = 256; through the spectral array itself every compute unit runs many wavefronts
   // and the extra pass is cheaper than the idle lanes

@@ -71,9 +71,12 @@ int memb_coeffs_f64_dev(memb_plan* p, int64_t ncap, const double* x, const doubl
 
 /* Geometry only (no law): the unit normal (x_theta x x_phi)/|.| and the point area on the
  * force grid, for capsules whose load comes from the surrogate.  The area equals the law's;
- * the normal is taken on the force grid instead of being filtered from the 2x grid. */
+ * the normal is taken on the force grid instead of being filtered from the 2x grid.  skip
+ * (optional device int array of ncap): capsules with skip[c] != 0 are not written (the law
+ * writes theirs). */
 int memb_geometry_f64_dev(memb_plan* p, int64_t ncap, const double* x, const double* y,
-                          const double* z, double* nx, double* ny, double* nz, double* area);
+                          const double* z, double* nx, double* ny, double* nz, double* area,
+                          const int* skip);
 
 /* The FP64 law for the capsules c with sel[c] != 0 only (device int array of ncap); the
  * outputs of the other capsules are left as they are.  The surrogate's fallback: capsules
@@ -103,12 +106,21 @@ int memb_corot_apply_f64_dev(memb_plan* p, int64_t ncap, const double* flin, con
  * complagrforces, forcing.f90:11-114).  Rows are [ncap][3][npts] FP64 in the co-rotated frame.
  * Prediction and gate: c = <delta, delta0> / |delta0|^2 (0 if delta0 = 0), f' = (lin + c^2 m0)
  * + b (m0 = NULL: lin + b), f = R f' into fx, fy, fz; sel[c] = 0 when beta * rms(delta -
- * delta0) <= tol and every value of f is finite, else 1; drift (optional, [ncap]) the rms. */
+ * delta0) <= tol and every value of f is finite, else 1; drift (optional, [ncap]) the rms.
+ * pre (optional): the drift test's result from memb_anchored_gate_f64_dev; capsules with
+ * pre[c] != 0 get sel[c] = 1 and nothing else is written for them (the law computes them).
+ * late (optional): late[c] = 1 for the other capsules rejected here (non-finite prediction),
+ * else 0: the law's second call. */
 int memb_anchored_predict_f64_dev(memb_plan* p, int64_t ncap, const double* lin,
                                   const double* m0, const double* b, const double* delta,
                                   const double* delta0, const double* rot, double beta,
                                   double tol, double* fx, double* fy, double* fz, int* sel,
-                                  double* drift);
+                                  double* drift, const int* pre, int* late);
+/* The drift test alone, as memb_anchored_predict_f64_dev computes it (same sums, same order):
+ * pre[c] = 0 when beta * rms(delta - delta0) <= tol, else 1.  It needs no linear response, so
+ * the law can start on the rejected capsules while J delta is computed for the others. */
+int memb_anchored_gate_f64_dev(memb_plan* p, int64_t ncap, const double* delta,
+                               const double* delta0, double beta, double tol, int* pre);
 /* New anchors for the capsules with sel != 0, after the law has written their traction into
  * fx, fy, fz: refreshed[c] += 1 (int64); b = R^T f - lin (- m_now), delta0 = delta, m0 = m_now
  * and s0 = amp (when m_now is given).  slot (optional): m_now has a row only for the capsules

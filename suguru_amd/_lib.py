@@ -174,11 +174,12 @@ def load_memb() -> ctypes.CDLL:
         "memb_ncoef": (I, [P]),
         "memb_traction_f64_dev": (I, [P, I64, D, D] + [P] * 10),
         "memb_coeffs_f64_dev": (I, [P, I64, P, P, P, P]),
-        "memb_geometry_f64_dev": (I, [P, I64, P, P, P, P, P, P, P]),
+        "memb_geometry_f64_dev": (I, [P, I64, P, P, P, P, P, P, P, P]),
         "memb_traction_select_f64_dev": (I, [P, I64, P, D, D] + [P] * 10),
         "memb_corot_f64_dev": (I, [P, I64] + [P] * 10),
         "memb_corot_apply_f64_dev": (I, [P, I64] + [P] * 7),
-        "memb_anchored_predict_f64_dev": (I, [P, I64] + [P] * 6 + [D, D] + [P] * 5),
+        "memb_anchored_predict_f64_dev": (I, [P, I64] + [P] * 6 + [D, D] + [P] * 7),
+        "memb_anchored_gate_f64_dev": (I, [P, I64, P, P, D, D, P]),
         "memb_anchored_update_f64_dev": (I, [P, I64] + [P] * 15),
         "memb_compact_sel_dev": (I, [P, I64, P, I, P, P]),
         "memb_mlp_create": (I, [ctypes.POINTER(P), I, I, I, I, I, I]),

@@ -878,18 +878,29 @@ __global__ void __launch_bounds__(256)
 //   cc = <delta, delta0> / |delta0|^2 (0 when delta0 = 0), drift = sqrt(sum (delta - delta0)^2 / n),
 //   f' = (lin + cc^2 m0) + b (m0 = NULL: f' = lin + b), f = R f' into fx, fy, fz,
 //   sel = 1 unless beta drift <= tol and every f is finite; drift_out (optional).
+// pre (optional): capsules with pre[c] != 0 (already rejected by the drift test, k_anch_gate)
+// are left to the law: sel[c] = 1, nothing else written.  late (optional): 1 for the other
+// capsules this kernel rejects (their prediction is not finite), else 0.
 __global__ void __launch_bounds__(CR_THREADS)
     k_anch_predict(int n, const double* __restrict__ lin, const double* __restrict__ m0,
                    const double* __restrict__ b, const double* __restrict__ delta,
                    const double* __restrict__ delta0, const double* __restrict__ rot,
                    double beta, double tol, double* __restrict__ fx, double* __restrict__ fy,
                    double* __restrict__ fz, int* __restrict__ sel,
-                   double* __restrict__ drift_out) {
+                   double* __restrict__ drift_out, const int* __restrict__ pre,
+                   int* __restrict__ late) {
   __shared__ double red[3][CR_THREADS / 64];
   __shared__ int bad;
   const long long cap = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const long long r0 = cap * 3 * n;
+  if (pre && pre[cap]) {
+    if (tid == 0) {
+      sel[cap] = 1;
+      if (late) late[cap] = 0;
+    }
+    return;
+  }
   if (tid == 0) bad = 0;
   double v[3] = {0.0, 0.0, 0.0};
   for (int q = tid; q < 3 * n; q += CR_THREADS) {
@@ -925,7 +936,34 @@ __global__ void __launch_bounds__(CR_THREADS)
   __syncthreads();
   if (tid == 0) {
     sel[cap] = (beta * drift <= tol && !bad) ? 0 : 1;
+    if (late) late[cap] = sel[cap];
     if (drift_out) drift_out[cap] = drift;
+  }
+}
+
+// The drift test of k_anch_predict alone (the same sums in the same order, so the same
+// drift): pre[c] = 1 unless beta rms(delta - delta0) <= tol.  The law can then start on these
+// capsules while the linear response is computed for the others.
+__global__ void __launch_bounds__(CR_THREADS)
+    k_anch_gate(int n, const double* __restrict__ delta, const double* __restrict__ delta0,
+                double beta, double tol, int* __restrict__ pre) {
+  __shared__ double red[CR_THREADS / 64];
+  const long long cap = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const long long r0 = cap * 3 * n;
+  double v = 0.0;
+  for (int q = tid; q < 3 * n; q += CR_THREADS) {
+    const double e = delta[r0 + q] - delta0[r0 + q];
+    v += e * e;
+  }
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  if (tid == 0) {
+    double sdr = 0.0;
+    for (int w = 0; w < CR_THREADS / 64; ++w) sdr += red[w];
+    const double drift = sqrt(sdr / (double)n);
+    pre[cap] = beta * drift <= tol ? 0 : 1;
   }
 }
 
@@ -1016,7 +1054,7 @@ constexpr int GM_THREADS = 256;
 __global__ void __launch_bounds__(GM_THREADS)
     k_geom(Tables T, long long ncap, const double* __restrict__ x, const double* __restrict__ y,
            const double* __restrict__ z, double* __restrict__ nx, double* __restrict__ ny,
-           double* __restrict__ nz, double* __restrict__ area) {
+           double* __restrict__ nz, double* __restrict__ area, const int* __restrict__ skip) {
   extern __shared__ double lds[];
   const int tid = threadIdx.x;
   const int N = T.N, NN = N * N, np2 = T.np2, nc = N * np2;
@@ -1029,6 +1067,7 @@ __global__ void __launch_bounds__(GM_THREADS)
   double* pts = tmp;                // [3][nc] the capsule's points (dead before tmp is written)
   for (int t = tid; t < np2 * N; t += GM_THREADS) c2[t] = T.cs2[t], s2[t] = T.sn2[t];
   for (long long cap = blockIdx.x; cap < ncap; cap += gridDim.x) {
+    if (skip && skip[cap]) continue;  // uniform over the workgroup
     // the capsule's points into LDS with coalesced loads
     for (int t = tid; t < 3 * nc; t += GM_THREADS) {
       const int c = t / nc, q = t - c * nc;
@@ -1287,7 +1326,8 @@ int memb_coeffs_f64_dev(memb_plan* p, int64_t ncap, const double* x, const doubl
 }
 
 int memb_geometry_f64_dev(memb_plan* p, int64_t ncap, const double* x, const double* y,
-                          const double* z, double* nx, double* ny, double* nz, double* area) {
+                          const double* z, double* nx, double* ny, double* nz, double* area,
+                          const int* skip) {
   if (!p) return MEMB_E_ARG;
   if (ncap < 0) return fail(p, MEMB_E_ARG, "ncap < 0");
   if (ncap > 0 && !(x && y && z && nx && ny && nz && area))
@@ -1297,7 +1337,7 @@ int memb_geometry_f64_dev(memb_plan* p, int64_t ncap, const double* x, const dou
   // two 256-thread workgroups per CU (LDS: 2 x 65 KB at L = 16)
   const unsigned nwg = (unsigned)std::min<long long>(ncap, (long long)p->num_wg / 2);
   k_geom<<<nwg, GM_THREADS, geom_lds_bytes(p->T), p->stream>>>(p->T, ncap, x, y, z, nx, ny, nz,
-                                                               area);
+                                                               area, skip);
   MCHK(p, hipGetLastError());
   return MEMB_OK;
 }
@@ -1334,7 +1374,7 @@ int memb_anchored_predict_f64_dev(memb_plan* p, int64_t ncap, const double* lin,
                                   const double* m0, const double* b, const double* delta,
                                   const double* delta0, const double* rot, double beta,
                                   double tol, double* fx, double* fy, double* fz, int* sel,
-                                  double* drift) {
+                                  double* drift, const int* pre, int* late) {
   if (!p) return MEMB_E_ARG;
   if (ncap < 0) return fail(p, MEMB_E_ARG, "ncap < 0");
   if (ncap > 0 && !(lin && b && delta && delta0 && rot && fx && fy && fz && sel))
@@ -1343,7 +1383,21 @@ int memb_anchored_predict_f64_dev(memb_plan* p, int64_t ncap, const double* lin,
   MCHK(p, hipSetDevice(p->device));
   const int n = p->T.N * p->T.np2;
   k_anch_predict<<<(unsigned)ncap, CR_THREADS, 0, p->stream>>>(n, lin, m0, b, delta, delta0, rot,
-                                                              beta, tol, fx, fy, fz, sel, drift);
+                                                              beta, tol, fx, fy, fz, sel, drift,
+                                                              pre, late);
+  MCHK(p, hipGetLastError());
+  return MEMB_OK;
+}
+
+int memb_anchored_gate_f64_dev(memb_plan* p, int64_t ncap, const double* delta,
+                               const double* delta0, double beta, double tol, int* pre) {
+  if (!p) return MEMB_E_ARG;
+  if (ncap < 0) return fail(p, MEMB_E_ARG, "ncap < 0");
+  if (ncap > 0 && !(delta && delta0 && pre)) return fail(p, MEMB_E_ARG, "null pointer");
+  if (ncap == 0) return MEMB_OK;
+  MCHK(p, hipSetDevice(p->device));
+  const int n = p->T.N * p->T.np2;
+  k_anch_gate<<<(unsigned)ncap, CR_THREADS, 0, p->stream>>>(n, delta, delta0, beta, tol, pre);
   MCHK(p, hipGetLastError());
   return MEMB_OK;
 }

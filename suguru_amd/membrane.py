@@ -61,10 +61,12 @@ class MembranePlan:
         self._check(self.lib.memb_coeffs_f64_dev(self.h, ncap, _ptr(x), _ptr(y), _ptr(z),
                                                  _ptr(coef)))
 
-    def geometry_dev(self, x, y, z, nx, ny, nz, area):
+    def geometry_dev(self, x, y, z, nx, ny, nz, area, skip=None):
+        """Normals and areas (memb_geometry_f64_dev); capsules with skip[c] != 0 untouched."""
         ncap = x.numel() // self.npts
         self._check(self.lib.memb_geometry_f64_dev(self.h, ncap, _ptr(x), _ptr(y), _ptr(z),
-                                                   _ptr(nx), _ptr(ny), _ptr(nz), _ptr(area)))
+                                                   _ptr(nx), _ptr(ny), _ptr(nz), _ptr(area),
+                                                   _ptr(skip)))
 
     def traction_select_dev(self, sel, x, y, z, gs, eb, fx, fy, fz, nx, ny, nz, area):
         """The law for the capsules with sel != 0 (int32 device tensor of ncap) only."""
@@ -86,12 +88,19 @@ class MembranePlan:
                                                       _ptr(fz)))
 
     def anchored_predict_dev(self, lin, m0, b, delta, delta0, rot, beta, tol, fx, fy, fz, sel,
-                             drift=None):
+                             drift=None, pre=None, late=None):
         """The anchored surrogate's prediction and gate (memb_anchored_predict_f64_dev)."""
         ncap = rot.numel() // 9
         self._check(self.lib.memb_anchored_predict_f64_dev(
             self.h, ncap, _ptr(lin), _ptr(m0), _ptr(b), _ptr(delta), _ptr(delta0), _ptr(rot),
-            float(beta), float(tol), _ptr(fx), _ptr(fy), _ptr(fz), _ptr(sel), _ptr(drift)))
+            float(beta), float(tol), _ptr(fx), _ptr(fy), _ptr(fz), _ptr(sel), _ptr(drift),
+            _ptr(pre), _ptr(late)))
+
+    def anchored_gate_dev(self, delta, delta0, beta, tol, pre):
+        """The drift test alone (memb_anchored_gate_f64_dev): pre[c] = 1 for rejected capsules."""
+        ncap = pre.numel()
+        self._check(self.lib.memb_anchored_gate_f64_dev(self.h, ncap, _ptr(delta), _ptr(delta0),
+                                                        float(beta), float(tol), _ptr(pre)))
 
     def anchored_update_dev(self, sel, slot, rot, fx, fy, fz, lin, m_now, delta, amp, b, delta0,
                             m0, s0, refreshed):
@@ -197,28 +206,6 @@ class SurrogateMLP:
         self._check(self.lib.memb_mlp_grad_only(self.h, int(batch), _ptr(x), _ptr(t), _ptr(wt),
                                                 ctypes.byref(loss)))
         return loss.value
-
-    def anchored_predict_dev(self, lin, m0, b, delta, delta0, rot, beta, tol, fx, fy, fz, sel,
-                             drift=None):
-        """The anchored surrogate's prediction and gate (memb_anchored_predict_f64_dev)."""
-        ncap = rot.numel() // 9
-        self._check(self.lib.memb_anchored_predict_f64_dev(
-            self.h, ncap, _ptr(lin), _ptr(m0), _ptr(b), _ptr(delta), _ptr(delta0), _ptr(rot),
-            float(beta), float(tol), _ptr(fx), _ptr(fy), _ptr(fz), _ptr(sel), _ptr(drift)))
-
-    def anchored_update_dev(self, sel, slot, rot, fx, fy, fz, lin, m_now, delta, amp, b, delta0,
-                            m0, s0, refreshed):
-        """New anchors for the selected capsules (memb_anchored_update_f64_dev)."""
-        ncap = rot.numel() // 9
-        self._check(self.lib.memb_anchored_update_f64_dev(
-            self.h, ncap, _ptr(sel), _ptr(slot), _ptr(rot), _ptr(fx), _ptr(fy), _ptr(fz),
-            _ptr(lin), _ptr(m_now), _ptr(delta), _ptr(amp), _ptr(b), _ptr(delta0), _ptr(m0),
-            _ptr(s0), _ptr(refreshed)))
-
-    def compact_sel_dev(self, sel, k, idx, slot):
-        """The first k selected capsules (memb_compact_sel_dev)."""
-        self._check(self.lib.memb_compact_sel_dev(self.h, int(sel.numel()), _ptr(sel), int(k),
-                                                  _ptr(idx), _ptr(slot)))
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:

@@ -496,9 +496,13 @@ class AnchoredTraction:
 
     learned = False drops the MLP term (m0 = 0): the same scheme without the learned part.
     The network runs only for the capsules whose anchor moves (their SH coefficients
-    gathered into max_refresh rows); geometry (normals, areas) runs on a side stream beside
-    the prediction, and the prediction, gate and anchor updates are fused kernels
-    (memb_anchored_predict_f64_dev / memb_anchored_update_f64_dev).
+    gathered into max_refresh rows).  The drift test needs no linear response, so it runs
+    first (memb_anchored_gate_f64_dev): the law then recomputes the capsules it rejects on a
+    second stream, and the geometry pass (normals, areas) the others on a third, while J
+    delta, the fused prediction (which skips the law's capsules and flags any non-finite
+    prediction for a second law call after the streams join) and the network run on the
+    surrogate's stream.  Every capsule gets the same values as with the law run after the
+    prediction.
     """
 
     def __init__(self, sur: Surrogate, ncap: int, beta: float, tol: float = 1e-3,
@@ -516,6 +520,7 @@ class AnchoredTraction:
         self.refreshed = t.zeros(ncap, dtype=t.int64, device=sur.dev)   # law calls per capsule
         self.steps = 0
         self._side = None   # stream of the geometry pass
+        self._lawst = None  # stream of the law on the drift-rejected capsules
 
     def reset(self):
         with self.s._streamed():
@@ -529,17 +534,30 @@ class AnchoredTraction:
         five times stricter and the law runs five times as often.)"""
         return delta @ self.s.JT
 
-    def _geometry(self, x, y, z, nx, ny, nz, area):
+    def _geometry(self, x, y, z, nx, ny, nz, area, skip=None):
         """Normals and areas on a side stream (the law's geometry pass), concurrent with the
-        prediction; joined before the selective law, which shares the plan's scratch."""
+        prediction; capsules with skip[c] != 0 are left to the law.  Joined before the
+        second law call, which writes normals and areas too."""
         s, t = self.s, self.s.torch
         if self._side is None:
             self._side = t.cuda.Stream(s.dev)
-        main = t.cuda.current_stream(s.dev)
-        self._side.wait_stream(main)
+        self._side.wait_stream(t.cuda.current_stream(s.dev))
         s.plan.set_stream(self._side.cuda_stream)
         try:
-            s.plan.geometry_dev(x, y, z, nx, ny, nz, area)
+            s.plan.geometry_dev(x, y, z, nx, ny, nz, area, skip)
+        finally:
+            s.plan.set_stream(s.stream_handle)
+
+    def _law_beside(self, sel, x, y, z, fx, fy, fz, nx, ny, nz, area):
+        """The FP64 law for the capsules with sel != 0 on a stream of its own (it writes only
+        their rows of the outputs and allocates nothing)."""
+        s, t = self.s, self.s.torch
+        if self._lawst is None:
+            self._lawst = t.cuda.Stream(s.dev)
+        self._lawst.wait_stream(t.cuda.current_stream(s.dev))
+        s.plan.set_stream(self._lawst.cuda_stream)
+        try:
+            s.plan.traction_select_dev(sel, x, y, z, s.gs, s.eb, fx, fy, fz, nx, ny, nz, area)
         finally:
             s.plan.set_stream(s.stream_handle)
 
@@ -569,28 +587,34 @@ class AnchoredTraction:
         """Traction, normals and areas of every capsule on the surrogate's stream.  Returns
         the int32 mask of the capsules the law recomputed this step (device)."""
         s, t = self.s, self.s.torch
+        i32 = lambda n: t.empty(n, dtype=t.int32, device=s.dev)
         with s._streamed():
-            self._geometry(x, y, z, nx, ny, nz, area)
-            sel = t.empty(self.ncap, dtype=t.int32, device=s.dev)
-            delta, dh, rot, amp, lin = self._predict_into(x, y, z, fx, fy, fz, sel)
-            t.cuda.current_stream(s.dev).wait_stream(self._side)
-            s.plan.traction_select_dev(sel, x, y, z, s.gs, s.eb, fx, fy, fz, nx, ny, nz, area)
+            main = t.cuda.current_stream(s.dev)
+            delta, dh, rot, amp = s.corot(x, y, z)
+            pre = i32(self.ncap)
+            s.plan.anchored_gate_dev(delta, self.delta0, self.beta, self.tol, pre)
+            self._geometry(x, y, z, nx, ny, nz, area, skip=pre)
+            self._law_beside(pre, x, y, z, fx, fy, fz, nx, ny, nz, area)
+            lin = self._linear(delta)
+            sel, late = i32(self.ncap), i32(self.ncap)
+            s.plan.anchored_predict_dev(lin, self.m0 if self.learned else None, self.b, delta,
+                                        self.delta0, rot, self.beta, self.tol, fx, fy, fz,
+                                        sel, None, pre, late)
+            slot = m_k = None
             if self.learned:
                 # the network's term for the (first kmax) capsules whose anchor moves
-                idx = t.empty(self.kmax, dtype=t.int32, device=s.dev)
-                slot = t.empty(self.ncap, dtype=t.int32, device=s.dev)
+                idx, slot = i32(self.kmax), i32(self.ncap)
                 s.plan.compact_sel_dev(sel, self.kmax, idx, slot)
                 il = idx.long()
                 dh_k = [d.view(self.ncap, -1)[il].reshape(-1) for d in dh]
                 a_k = amp[il]
                 m_k = (a_k * a_k)[:, None] * s.predict(s.features(dh_k)).double()
-                s.plan.anchored_update_dev(sel, slot, rot, fx, fy, fz, lin, m_k, delta, amp,
-                                           self.b, self.delta0, self.m0, self.s0,
-                                           self.refreshed)
-            else:
-                s.plan.anchored_update_dev(sel, None, rot, fx, fy, fz, lin, None, delta, amp,
-                                           self.b, self.delta0, self.m0, self.s0,
-                                           self.refreshed)
+            main.wait_stream(self._lawst)
+            main.wait_stream(self._side)
+            # non-finite predictions: the law for them too
+            s.plan.traction_select_dev(late, x, y, z, s.gs, s.eb, fx, fy, fz, nx, ny, nz, area)
+            s.plan.anchored_update_dev(sel, slot, rot, fx, fy, fz, lin, m_k, delta, amp,
+                                       self.b, self.delta0, self.m0, self.s0, self.refreshed)
         self.steps += 1
         return sel
 

@@ -127,16 +127,28 @@ __device__ __forceinline__ int brev(int n, int logm) {
   return logm ? (int)(__brev((unsigned)n) >> (32 - logm)) : 0;
 }
 
+// The velocity divergence of fillps.f90:8-47 as the source of an x transform: the value of
+// compact cell (i, j, k) is computed from the fields while the lines are loaded, with
+// k_fillps's expression (cz = dzi / dt, cy = dyi / dt, cx = dxi / dt as k_fillps forms them).
+struct FillSrc {
+  const double* u;
+  const double* v;
+  const double* w;
+  double cx, cy, cz;
+  long long sy, sz;
+};
+
 // Forward: real lines -> packed half spectra.  XMAJOR: element index fastest across threads
 // (x lines); otherwise line index fastest (y lines).  src and dst may be the same array (the
 // workgroup reads all of its lines before the first barrier and writes them after the last).
-template <bool XMAJOR>
+// FILL (x lines of a compact array only): the lines are fillps's divergence (fs), not src.
+template <bool XMAJOR, bool FILL = false>
 __global__ void __launch_bounds__(FFT_THREADS)
     k_rfft_fwd(const double* src, double* dst, int N, int logm,
                int nl, int groups_per_plane, long long plane_src, long long plane_dst,
                long long gstride_src, long long gstride_dst, long long es_src, long long ls_src,
                long long es_dst, long long ls_dst, long long off_src,
-               const double2* __restrict__ tw) {
+               const double2* __restrict__ tw, FillSrc fs) {
   extern __shared__ double2 sm[];
   const int M = N >> 1, logn = logm + 1, lognl = 31 - __clz(nl);
   const int blk = xcd_logical_block<XMAJOR>();
@@ -161,7 +173,15 @@ __global__ void __launch_bounds__(FFT_THREADS)
       const int q = q0 + u * blockDim.x;
       const int l = XMAJOR ? q >> logn : q & (nl - 1), e = XMAJOR ? q & (N - 1) : q >> lognl;
       v[u] = 0.0;
-      if (q < nl * N) v[u] = src[bs + l * ls_src + e * es_src];
+      if (q < nl * N) {
+        if constexpr (FILL) {
+          const long long c = (long long)(e + 1) + fs.sy * (gp * nl + l + 1) + fs.sz * (kp + 1);
+          v[u] = ((fs.w[c] - fs.w[c - fs.sz]) * fs.cz + (fs.v[c] - fs.v[c - fs.sy]) * fs.cy +
+                  (fs.u[c] - fs.u[c - 1]) * fs.cx);
+        } else {
+          v[u] = src[bs + l * ls_src + e * es_src];
+        }
+      }
     }
 #pragma unroll
     for (int u = 0; u < FFT_LB_FWD; ++u) {
@@ -981,7 +1001,7 @@ int fft_x(fld_ctx* c, bool inv, const double* src, bool src_field, double* dst, 
   if (!inv)
     k_rfft_fwd<true><<<nb, FFT_THREADS, lds, c->stream>>>(
         src, dst, N, c->logmx, nl, gpp, plane_s, plane_d, nl * row_s, nl * row_d, 1, row_s, 1,
-        row_d, src_field ? off : 0, c->tw_x);
+        row_d, src_field ? off : 0, c->tw_x, FillSrc{});
   else
     k_rfft_inv<true><<<nb, FFT_THREADS, lds, c->stream>>>(
         src, dst, N, c->logmx, nl, gpp, plane_s, plane_d, nl * row_s, nl * row_d, 1, row_s, 1,
@@ -999,7 +1019,7 @@ int fft_y(fld_ctx* c, bool inv, double* a) {
   const unsigned nb = (unsigned)(gpp * g.ktot);
   if (!inv)
     k_rfft_fwd<false><<<nb, FFT_THREADS, c->lds_yf, c->stream>>>(
-        a, a, N, c->logmy, nl, gpp, cp, cp, nl, nl, g.itot, 1, g.itot, 1, 0, c->tw_y);
+        a, a, N, c->logmy, nl, gpp, cp, cp, nl, nl, g.itot, 1, g.itot, 1, 0, c->tw_y, FillSrc{});
   else
     k_rfft_inv<false><<<nb, FFT_THREADS, c->lds_yi, c->stream>>>(
         a, a, N, c->logmy, nl, gpp, cp, cp, nl, nl, g.itot, 1, g.itot, 1, 0, c->tw_y);
@@ -1026,12 +1046,38 @@ int tri(fld_ctx* c, double* a, int ne) {
   return FLD_OK;
 }
 
+// solver1d after the forward x transform: y transform, eliminations, inverses into the
+// interior of `out`
+int solve_after_x(fld_ctx* c, double* a, int ne, double* out) {
+  int rc;
+  if ((rc = fft_y(c, false, a)) || (rc = tri(c, a, ne)) || (rc = fft_y(c, true, a)) ||
+      (rc = fft_x(c, true, a, false, out, true)))
+    return rc;
+  return FLD_OK;
+}
+
 // solver1d on a compact right-hand side, result into the interior of `out`
 int solve_compact(fld_ctx* c, double* a, int ne, double* out) {
   int rc;
-  if ((rc = fft_x(c, false, a, false, a, false)) || (rc = fft_y(c, false, a)) ||
-      (rc = tri(c, a, ne)) || (rc = fft_y(c, true, a)) || (rc = fft_x(c, true, a, false, out, true)))
-    return rc;
+  if ((rc = fft_x(c, false, a, false, a, false))) return rc;
+  return solve_after_x(c, a, ne, out);
+}
+
+// fillps (the divergence of u, v, w over dt) already transformed in x, into the compact
+// array dst: k_fillps and the pressure solve's forward x transform in one pass
+int fft_x_fill(fld_ctx* c, const double* u, const double* v, const double* w, double* dst) {
+  const FGrid& g = c->g;
+  const int N = g.itot, nl = c->xlines;
+  const int gpp = g.jtot / nl;
+  const long long cp = (long long)g.itot * g.jtot, row = g.itot;
+  const size_t lds = (size_t)nl * (N / 2 + 1) * sizeof(double2);
+  const unsigned nb = (unsigned)(gpp * g.ktot);
+  const double dti = 1.0 / c->dt;
+  const FillSrc fs{u, v, w, dti * g.dxi, dti * g.dyi, dti * g.dzi, g.sy, g.sz};
+  k_rfft_fwd<true, true><<<nb, FFT_THREADS, lds, c->stream>>>(
+      nullptr, dst, N, c->logmx, nl, gpp, cp, cp, nl * row, nl * row, 1, row, 1, row, 0, c->tw_x,
+      fs);
+  HIPF(c, hipGetLastError());
   return FLD_OK;
 }
 
@@ -1102,6 +1148,8 @@ int fld_create(fld_ctx** out, const ibx_grid* gr, double rep, double dt, int dev
                           (int)LDS_MAX) != hipSuccess ||
       hipFuncSetAttribute((const void*)k_rfft_fwd<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)LDS_MAX) != hipSuccess ||
+      hipFuncSetAttribute((const void*)k_rfft_fwd<true, true>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_MAX) != hipSuccess ||
       hipFuncSetAttribute((const void*)k_rfft_inv<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)LDS_MAX) != hipSuccess ||
       hipFuncSetAttribute((const void*)k_tri<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1257,9 +1305,8 @@ int fld_step_dev(fld_ctx* c, double* unew, double* vnew, double* wnew, double* p
   int rc = fld_fftuvw_dev(c, unew, vnew, wnew, uo, vo, wo, uoo, voo, woo, pnew, fx, fy, fz);
   if (rc) return rc;
   if ((rc = bound(c, unew, vnew, wnew, 3, 0))) return rc;
-  k_fillps<<<nblocks(c->ncomp, 256), 256, 0, c->stream>>>(c->g, c->dt, unew, vnew, wnew, c->work[0]);
-  HIPF(c, hipGetLastError());
-  if ((rc = solve_compact(c, c->work[0], 1, p)) || (rc = bound(c, p, nullptr, nullptr, 1, 2)))
+  if ((rc = fft_x_fill(c, unew, vnew, wnew, c->work[0])) ||
+      (rc = solve_after_x(c, c->work[0], 1, p)) || (rc = bound(c, p, nullptr, nullptr, 1, 2)))
     return rc;
   // correc and pnew += p fused (the reference's bounduvw between them reads neither p nor
   // pnew), then bounduvw and boundp(pnew)

@@ -5,8 +5,8 @@
 // in compact spectral arrays of itot x jtot x ktot doubles (i fastest), one per velocity
 // component; the pressure solve reuses the first.  A solve is five passes:
 //   1. x transform of every (j, k) line (k_rfft, lines contiguous),
-//   2. y transform of every (i, k) line (k_rfft, lines = 4 (forward) or 8 (inverse)
-//      neighbouring columns per workgroup: row segments of 32 or 64 bytes),
+//   2. y transform of every (i, k) line (k_rfft, 8 neighbouring columns per workgroup: row
+//      segments of 64 bytes),
 //   3. Gaussian elimination in k per (i, j) mode (k_tri_seg: one lane per mode, the k rows in
 //      register-held segments joined through LDS, the per-mode reciprocal pivots read from a
 //      table built once per context; k_tri, two passes through the array, above 257 planes),
@@ -37,7 +37,7 @@ constexpr int FFT_THREADS = 256;
 constexpr int FFT_LB_FWD = 8;
 constexpr int FFT_LB_INV = 4;
 constexpr int TRI_MODES = 64;        // modes (lanes) per elimination workgroup
-constexpr int Y_LINES_FWD = 4;       // columns per forward y-transform workgroup
+constexpr int Y_LINES_FWD = 8;       // columns per forward y-transform workgroup
 constexpr int Y_LINES_INV = 8;       // columns per inverse y-transform workgroup
 constexpr size_t LDS_MAX = 160 * 1024;
 
@@ -1115,8 +1115,8 @@ int fld_create(fld_ctx** out, const ibx_grid* gr, double rep, double dt, int dev
   c->ncomp = (long long)g.itot * g.jtot * g.ktot;
   c->logmx = ilog2(g.itot / 2), c->logmy = ilog2(g.jtot / 2);
   c->xlines = std::min(g.jtot, std::max(1, 2048 / g.itot));
-  // the forward y transform runs more workgroups of fewer columns (4: 760 us against 881 us
-  // with 8 at 1024 x 512 x 256); the inverse is fastest with 8
+  // 8 columns per workgroup both ways (at 1024 x 512 x 256: forward 543 us against 604 with 4
+  // and 744 with 2; inverse 602 us against 674 with 16)
   auto lines = [&](int want) {
     int nl = std::min(g.itot, want);
     while (nl > 1 && (size_t)nl * (g.jtot / 2 + 1) * sizeof(double2) > LDS_MAX / 2) nl >>= 1;

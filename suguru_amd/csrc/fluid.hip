@@ -307,13 +307,12 @@ __global__ void __launch_bounds__(256) k_ztab(int nkx, int nky, int ktot, int ne
   ztab[(long long)(ktot - 1) * st + t] = bb - a[ktot - 1] * d;
 }
 
-// One lane per (i, j) mode of the compact spectral array.  LDS: the forward sweep's values,
-// TRI_MODES x ktot doubles (else they go back to the array and are read again).
-template <bool LDS>
+// One lane per (i, j) mode of the compact spectral array, the forward sweep's values stored
+// back into the array and read again by the back substitution (the path above 257 planes;
+// k_tri_seg below takes every smaller grid in one pass).
 __global__ void __launch_bounds__(TRI_MODES)
     k_tri(double* __restrict__ p, int itot, int jtot, int ktot, const double* __restrict__ a,
           const double* __restrict__ c, const double* __restrict__ ztab) {
-  extern __shared__ double fw[];
   const long long plane = (long long)itot * jtot;
   const long long m = (long long)blockIdx.x * TRI_MODES + threadIdx.x;
   if (m >= plane) return;
@@ -325,18 +324,14 @@ __global__ void __launch_bounds__(TRI_MODES)
   for (int k = 0; k < ktot - 1; ++k) {
     const double z = ztab[k * zs + zb];
     prev = (p[k * plane + m] - a[k] * prev) * z;
-    if (LDS)
-      fw[k * TRI_MODES + threadIdx.x] = prev;
-    else
-      p[k * plane + m] = prev;
+    p[k * plane + m] = prev;
   }
   const double zl = ztab[(long long)(ktot - 1) * zs + zb];
   double nxt = zl != 0.0 ? (p[(long long)(ktot - 1) * plane + m] - a[ktot - 1] * prev) / zl : 0.0;
   p[(long long)(ktot - 1) * plane + m] = nxt;
   for (int k = ktot - 2; k >= 0; --k) {
     const double d = c[k] * ztab[k * zs + zb];
-    const double v = LDS ? fw[k * TRI_MODES + threadIdx.x] : p[k * plane + m];
-    nxt = v - d * nxt;
+    nxt = p[k * plane + m] - d * nxt;
     p[k * plane + m] = nxt;
   }
 }
@@ -899,7 +894,6 @@ struct fld_ctx {
   double2* tw_y = nullptr;
   double* part = nullptr;  // chkdiv partials
   int logmx = 0, logmy = 0;
-  bool tri_lds = true;
   size_t lds_yf = 0, lds_yi = 0;
   int ylines_f = Y_LINES_FWD, ylines_i = Y_LINES_INV, xlines = 1;
 };
@@ -1036,12 +1030,9 @@ int tri(fld_ctx* c, double* a, int ne) {
   if (nseg <= TRS_MAXSEG)
     k_tri_seg<<<nblocks(modes, TRI_MODES), TRI_MODES * nseg, 0, c->stream>>>(
         a, g.itot, g.jtot, g.ktot, ac, cc, c->ztab[ne]);
-  else if (c->tri_lds)
-    k_tri<true><<<nblocks(modes, TRI_MODES), TRI_MODES, (size_t)TRI_MODES * g.ktot * sizeof(double),
-                  c->stream>>>(a, g.itot, g.jtot, g.ktot, ac, cc, c->ztab[ne]);
   else
-    k_tri<false><<<nblocks(modes, TRI_MODES), TRI_MODES, 0, c->stream>>>(a, g.itot, g.jtot, g.ktot,
-                                                                          ac, cc, c->ztab[ne]);
+    k_tri<<<nblocks(modes, TRI_MODES), TRI_MODES, 0, c->stream>>>(a, g.itot, g.jtot, g.ktot, ac,
+                                                                  cc, c->ztab[ne]);
   HIPF(c, hipGetLastError());
   return FLD_OK;
 }
@@ -1126,10 +1117,6 @@ int fld_create(fld_ctx** out, const ibx_grid* gr, double rep, double dt, int dev
   // lines plus the LDS copy of the twiddles
   c->lds_yf = ((size_t)c->ylines_f * (g.jtot / 2 + 1) + g.jtot / 2) * sizeof(double2);
   c->lds_yi = ((size_t)c->ylines_i * (g.jtot / 2 + 1) + g.jtot / 2) * sizeof(double2);
-  // the sweep's forward values in LDS limit a compute unit to one 64-lane workgroup at
-  // ktot = 256; through the spectral array itself every compute unit runs many wavefronts
-  // and the extra pass is cheaper than the idle lanes
-  c->tri_lds = false;
   auto bail = [&](int code, const char* what) {
     fld_destroy(c);
     (void)what;
@@ -1151,8 +1138,6 @@ int fld_create(fld_ctx** out, const ibx_grid* gr, double rep, double dt, int dev
       hipFuncSetAttribute((const void*)k_rfft_fwd<true, true>,
                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_MAX) != hipSuccess ||
       hipFuncSetAttribute((const void*)k_rfft_inv<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                          (int)LDS_MAX) != hipSuccess ||
-      hipFuncSetAttribute((const void*)k_tri<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)LDS_MAX) != hipSuccess)
     return bail(FLD_E_HIP, "lds");
   // twiddles, eigenvalues, coefficients and the pivot tables of e = 1, 2, 3

@@ -62,7 +62,10 @@ def solvers():
         s.close()
 
 
-@pytest.mark.parametrize("shape", [(16, 8, 12), (32, 16, 24), (8, 64, 9), (128, 64, 48)])
+# (8, 16, 300): more than 257 planes, the two-pass elimination; the others the segmented
+# one with 1, 2 and 3 segments
+@pytest.mark.parametrize("shape", [(16, 8, 12), (32, 16, 24), (8, 64, 9), (128, 64, 48),
+                                   (8, 16, 300)])
 @pytest.mark.parametrize("ne", [1, 2, 3])
 def test_solver1d_matches_oracle(solvers, shape, ne):
     torch = _torch()

@@ -147,6 +147,13 @@ void mem_legendre_derivs(int nmax, double theta, double* D) {
   free(P);
 }
 
+/* cos / sin(m phi_j), phi_j = 2 pi j / n, with the angle reduced first to 2 pi ((m j) mod n) / n:
+ * SPHEREPACK's longitude transforms are real FFTs whose twiddle factors are cos / sin of
+ * angles below 2 pi (hrfft.f:214-267, hrfti1), never of m phi_j itself, whose rounding
+ * (1e-14 at m phi_j ~ 2 pi L) the fourth phi derivatives multiply by m^4. */
+static inline double cos_mj(int m, int j, int n) { return cos(2.0 * M_PI * ((m * j) % n) / n); }
+static inline double sin_mj(int m, int j, int n) { return sin(2.0 * M_PI * ((m * j) % n) / n); }
+
 /* Analysis on an nt x np Gauss grid (f[i * np + j]) with the Pbar table P[i][m][n]
  * (degrees < nmax <= nt): a[m * nmax + n], b[m * nmax + n]. */
 static void analysis(int nt, int np, const double* w, const double* P, int nmax, const double* f,
@@ -157,9 +164,8 @@ static void analysis(int nt, int np, const double* w, const double* P, int nmax,
     for (int m = 0; m < nmax; ++m) {
       double cs = 0.0, sn = 0.0;
       for (int j = 0; j < np; ++j) {
-        const double ph = 2.0 * M_PI * j / np;
-        cs += f[i * np + j] * cos(m * ph);
-        sn += f[i * np + j] * sin(m * ph);
+        cs += f[i * np + j] * cos_mj(m, j, np);
+        sn += f[i * np + j] * sin_mj(m, j, np);
       }
       for (int n = m; n < nmax; ++n) {
         const double p = w[i] * P[((size_t)i * nmax + m) * nmax + n];
@@ -174,10 +180,9 @@ static void synthesis(int nt, int np, const double* P, int nmax, const double* a
                       double* f) {
   for (int i = 0; i < nt; ++i)
     for (int j = 0; j < np; ++j) {
-      const double ph = 2.0 * M_PI * j / np;
       double s = 0.0;
       for (int m = 0; m < nmax; ++m) {
-        const double cm = cos(m * ph), sm = sin(m * ph);
+        const double cm = cos_mj(m, j, np), sm = sin_mj(m, j, np);
         for (int n = m; n < nmax; ++n) {
           const double p = P[((size_t)i * nmax + m) * nmax + n] * (m == 0 ? 0.5 : 1.0);
           s += p * (a[m * nmax + n] * cm - b[m * nmax + n] * sm);
@@ -214,11 +219,10 @@ static void derivs_fine(const mem_plan* p, const double* a3, const double* b3, d
           A[k][m] = sa, B[k][m] = sb;
         }
       for (int j = 0; j < p->np; ++j) {
-        const double ph = 2.0 * M_PI * j / p->np;
         double* o = out + ((size_t)(i * p->np + j) * 3 + c) * NQ;
         for (int q = 0; q < NQ; ++q) o[q] = 0.0;
         for (int m = 0; m < N; ++m) {
-          const double cm = cos(m * ph), sm = sin(m * ph);
+          const double cm = cos_mj(m, j, p->np), sm = sin_mj(m, j, p->np);
           /* k_phi-th phi derivative of (A cos m phi - B sin m phi) */
           for (int kt = 0; kt <= 4; ++kt)
             for (int kp = 0; kt + kp <= 4; ++kp) {
@@ -655,11 +659,10 @@ int mem_traction(int L, double gs, double eb, const double* x, const double* y, 
   /* surface metric on the force grid from the coarse coefficients (sph3.f90:186-260) */
   for (int i = 0; i < N; ++i)
     for (int j = 0; j < p->np2; ++j) {
-      const double ph = 2.0 * M_PI * j / p->np2;
       double t1[3] = {0, 0, 0}, t2[3] = {0, 0, 0};
       for (int c = 0; c < 3; ++c)
         for (int m = 0; m < N; ++m) {
-          const double cm = cos(m * ph), sm = sin(m * ph), h = m == 0 ? 0.5 : 1.0;
+          const double cm = cos_mj(m, j, p->np2), sm = sin_mj(m, j, p->np2), h = m == 0 ? 0.5 : 1.0;
           for (int n = m; n < N; ++n) {
             const double an = a3[c * N * N + m * N + n], bn = b3[c * N * N + m * N + n];
             t1[c] += h * p->Pcd[((size_t)i * N + m) * N + n] * (an * cm - bn * sm);

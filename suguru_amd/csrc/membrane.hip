@@ -368,7 +368,8 @@ __device__ void law(const Geom& o, const double* __restrict__ R, double gs, doub
 // MODE 0: the law; 1: the reference state of one capsule (the unit sphere) into `ref`;
 // 2: shape coefficients only; 3: geometry only (unit normal and point area on the force
 // grid, no load).  Persistent: workgroup b handles capsules b, b + gridDim.x, ...
-// LDS: coarse coefficients [3][N][N] a and b, then a work region reused by the phases.
+// LDS: coarse coefficients [3][N][N] a and b, the longitude factors of both grids (2 (np + np2)
+// values), then a work region reused by the phases.
 // scratch: per workgroup, longitude DFT of the 2x-grid outputs [nt][6][N][2].
 // Registers: capped for two wavefronts per SIMD (256 VGPRs, about 40 of them spilled).  With
 // no cap the law takes 256 VGPRs + 40 AGPRs, one wavefront per SIMD, and runs 1.6x slower.
@@ -389,11 +390,21 @@ __global__ void __launch_bounds__(THREADS, 2)
   extern __shared__ double lds[];
   const int tid = threadIdx.x;
   const int N = T.N, NN = N * N, np2 = T.np2, nt = T.nt, np = T.np;
-  const int nc = N * np2;
+  const int nc = N * np2, NT = N * (N + 1) / 2;  // NT: pairs (m, n >= m)
   double* ca = lds;            // [3][N][N]
   double* cb = ca + 3 * NN;    // [3][N][N]
-  double* U = cb + 3 * NN;     // work region
+  // cos / sin(2 pi q / np) for q < np, then the same for q < np2: every longitude factor
+  // cos / sin(m phi_j) of the fine (force) grid is entry (m j) mod np (np2) of these, so the
+  // transforms read them from LDS instead of from the [j][m] tables in HBM
+  double* tcf = cb + 3 * NN;
+  double* tsf = tcf + np;
+  double* tc2 = tsf + np;
+  double* ts2 = tc2 + np2;
+  double* U = ts2 + np2;       // work region
   double* scr = scratch + (size_t)blockIdx.x * nt * 6 * N * 2;  // >= 6 nc doubles too
+  for (int q = tid; q < np; q += THREADS) tcf[q] = T.csf[q * N + 1], tsf[q] = T.snf[q * N + 1];
+  for (int q = tid; q < np2; q += THREADS) tc2[q] = T.cs2[q * N + 1], ts2[q] = T.sn2[q * N + 1];
+  __syncthreads();
   for (long long cap = blockIdx.x; cap < ncap; cap += gridDim.x) {
     if (sel && sel[cap] == 0) continue;  // (uniform in the workgroup)
     const double* xyz[3] = {x + cap * nc, y + cap * nc, z + cap * nc};
@@ -402,28 +413,39 @@ __global__ void __launch_bounds__(THREADS, 2)
       const int c = t / NN, i = (t / N) % N, m = t % N;
       const double* f = xyz[c] + i * np2;
       double cs = 0.0, sn = 0.0;
+      int q = 0;  // (m j) mod np2
       #pragma unroll 4
       for (int j = 0; j < np2; ++j) {
         const double v = f[j];
-        cs += v * T.cs2[j * N + m];
-        sn += v * T.sn2[j * N + m];
+        cs += v * tc2[q];
+        sn += v * ts2[q];
+        q += m;
+        if (q >= np2) q -= np2;
       }
       U[2 * t] = cs, U[2 * t + 1] = sn;
     }
     __syncthreads();
-    for (int t = tid; t < 3 * NN; t += THREADS) {
-      const int c = t / NN, m = (t / N) % N, n = t % N;
-      double sa = 0.0, sb = 0.0;
-      if (n >= m)
-        #pragma unroll 4
-        for (int i = 0; i < N; ++i) {
-          const double p = T.w2[i] * T.Pc[(i * N + m) * N + n];
+    // one thread per (m, n >= m) for the three coordinates (entries n < m are never read)
+    for (int t = tid; t < NT; t += THREADS) {
+      int m = 0, n = t;
+      while (n >= N - m) n -= N - m, ++m;
+      n += m;
+      double sa[3] = {0.0, 0.0, 0.0}, sb[3] = {0.0, 0.0, 0.0};
+      #pragma unroll 4
+      for (int i = 0; i < N; ++i) {
+        const double p = T.w2[i] * T.Pc[(i * N + m) * N + n];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
           const int u = (c * N + i) * N + m;
-          sa += p * U[2 * u];
-          sb += p * U[2 * u + 1];
+          sa[c] += p * U[2 * u];
+          sb[c] += p * U[2 * u + 1];
         }
-      ca[(c * N + m) * N + n] = 2.0 / np2 * sa;
-      cb[(c * N + m) * N + n] = -2.0 / np2 * sb;
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        ca[(c * N + m) * N + n] = 2.0 / np2 * sa[c];
+        cb[(c * N + m) * N + n] = -2.0 / np2 * sb[c];
+      }
     }
     __syncthreads();
     if (MODE == 2) {
@@ -452,6 +474,7 @@ __global__ void __launch_bounds__(THREADS, 2)
         const double* Pd = T.Pfd + (((size_t)(i0 + r) * 5 + k) * N + m) * N;
         const double h = m == 0 ? 0.5 : 1.0;
         double sa0 = 0.0, sb0 = 0.0, sa1 = 0.0, sb1 = 0.0, sa2 = 0.0, sb2 = 0.0;
+        #pragma unroll 4
         for (int n = m; n < N; ++n) {
           const double p = Pd[n];
           sa0 += p * ca[m * N + n], sb0 += p * cb[m * N + n];
@@ -471,8 +494,11 @@ __global__ void __launch_bounds__(THREADS, 2)
         for (int c = 0; c < 3; ++c)
 #pragma unroll
           for (int q = 0; q < NQ; ++q) X[c][q] = 0.0;
+        int q = 0;  // (m j) mod np
         for (int m = 0; m < N; ++m) {
-          const double cm = T.csf[j * N + m], sm = T.snf[j * N + m];
+          const double cm = tcf[q], sm = tsf[q];
+          q += j;
+          if (q >= np) q -= np;
           const double m1 = (double)m, m2 = m1 * m1, m3 = m2 * m1, m4 = m2 * m2;
 #pragma unroll
           for (int c = 0; c < 3; ++c)
@@ -516,9 +542,13 @@ __global__ void __launch_bounds__(THREADS, 2)
         for (int t = tid; t < nr * 2 * N; t += THREADS) {
           const int r = t / (2 * N), f0 = 3 * ((t / N) % 2), m = t % N;
           double cs0 = 0.0, sn0 = 0.0, cs1 = 0.0, sn1 = 0.0, cs2 = 0.0, sn2 = 0.0;
+          int q = 0;  // (m j) mod np
+          #pragma unroll 4
           for (int j = 0; j < np; ++j) {
             const double* v = F + ((size_t)r * np + j) * 6 + f0;
-            const double c = T.csf[j * N + m], s = T.snf[j * N + m];
+            const double c = tcf[q], s = tsf[q];
+            q += m;
+            if (q >= np) q -= np;
             cs0 += v[0] * c, sn0 += v[0] * s;
             cs1 += v[1] * c, sn1 += v[1] * s;
             cs2 += v[2] * c, sn2 += v[2] * s;
@@ -533,82 +563,119 @@ __global__ void __launch_bounds__(THREADS, 2)
     }
     if (MODE == 1) continue;
     // 3. filter: quadrature of the 2x-grid outputs (degrees <= L), synthesis on the force
-    //    grid, field by field; the area from the coarse shape (sph3.f90:186-260)
-    double* fc = U;              // [N][N][2] coefficients of the current field
-    double* tmp = U + 2 * NN;    // [N][N][2] per (i, m) partial sums
-    // the force grid's longitude tables [np2][N], staged once per capsule: the node loop of
-    // each of the twelve fields then waits on LDS instead of on two table loads per term
-    double* c2 = U + 4 * NN;
-    double* s2 = c2 + np2 * N;
-    for (int t = tid; t < np2 * N; t += THREADS) c2[t] = T.cs2[t], s2[t] = T.sn2[t];
+    //    grid, two fields at a time (each table load feeds both); the area from the coarse
+    //    shape (sph3.f90:186-260)
+    // fc: the six fields' coefficients, packed by (m, n >= m) at pk(m) + n - m with
+    // pk(m) = m N - m (m - 1) / 2, [6][N (N + 1) / 2][2]; tmp: per (i, m) partial sums of the
+    // current pair of fields
+    double* fc = U;
+    double* tmp = U + 12 * NT;   // [2][N][N][2]
     double* xt = scr;            // [nc][3] x_theta on the force grid (scratch reused)
     double* xp = scr + (size_t)nc * 3;  // [nc][3] x_phi
-    for (int fi = MODE == 3 ? 6 : 0; fi < 12; ++fi) {
-      if (fi < 6) {
-        for (int t = tid; t < NN; t += THREADS) {
-          const int m = t / N, n = t % N;
-          double sa = 0.0, sb = 0.0;
-          if (n >= m)
-            #pragma unroll 4
-            for (int i = 0; i < nt; ++i) {
-              const double p = T.w[i] * T.Pfa[((size_t)i * N + m) * N + n];
-              const double* s = scr + (((size_t)i * 6 + fi) * N + m) * 2;
-              sa += p * s[0];
-              sb += p * s[1];
-            }
-          fc[2 * t] = 2.0 / np * sa, fc[2 * t + 1] = -2.0 / np * sb;
+    if (MODE == 0) {
+      // quadrature of all six fields in one pass, one thread per (m, n >= m): each table load
+      // feeds twelve sums
+      for (int t = tid; t < NT; t += THREADS) {
+        int m = 0, n = t;
+        while (n >= N - m) n -= N - m, ++m;
+        n += m;
+        double sa[6], sb[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) sa[f] = 0.0, sb[f] = 0.0;
+        #pragma unroll 2
+        for (int i = 0; i < nt; ++i) {
+          const double p = T.w[i] * T.Pfa[((size_t)i * N + m) * N + n];
+          const double* s = scr + (((size_t)i * 6) * N + m) * 2;
+#pragma unroll
+          for (int f = 0; f < 6; ++f) {
+            sa[f] += p * s[2 * f * N];
+            sb[f] += p * s[2 * f * N + 1];
+          }
         }
-        __syncthreads();
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+          fc[2 * (f * NT + t)] = 2.0 / np * sa[f], fc[2 * (f * NT + t) + 1] = -2.0 / np * sb[f];
+      }
+      __syncthreads();
+    }
+    for (int f2 = MODE == 3 ? 6 : 0; f2 < 12; f2 += 2) {
+      if (f2 < 6) {
+        const double* fa = fc + 2 * f2 * NT;
+        const double* fb = fa + 2 * NT;
         for (int t = tid; t < NN; t += THREADS) {
           const int i = t / N, m = t % N;
           const double h = m == 0 ? 0.5 : 1.0;
-          double sa = 0.0, sb = 0.0;
+          const int pk = m * N - m * (m - 1) / 2 - m;  // + n
+          double sa0 = 0.0, sb0 = 0.0, sa1 = 0.0, sb1 = 0.0;
           #pragma unroll 4
           for (int n = m; n < N; ++n) {
             const double p = T.Pc[(i * N + m) * N + n];
-            sa += p * fc[2 * (m * N + n)];
-            sb += p * fc[2 * (m * N + n) + 1];
+            const int u = pk + n;
+            sa0 += p * fa[2 * u];
+            sb0 += p * fa[2 * u + 1];
+            sa1 += p * fb[2 * u];
+            sb1 += p * fb[2 * u + 1];
           }
-          tmp[2 * t] = h * sa, tmp[2 * t + 1] = h * sb;
+          tmp[2 * t] = h * sa0, tmp[2 * t + 1] = h * sb0;
+          tmp[2 * (NN + t)] = h * sa1, tmp[2 * (NN + t) + 1] = h * sb1;
         }
       } else {
         // fi 6..8: x_theta, fi 9..11: x_phi of coordinate c, from the coarse shape
-        const bool dth = fi < 9;
-        const int c = dth ? fi - 6 : fi - 9;
+        const int c0 = f2 < 9 ? f2 - 6 : f2 - 9, c1 = f2 + 1 < 9 ? f2 - 5 : f2 - 8;
+        const double* P0 = f2 < 9 ? T.Pcd : T.Pc;
+        const double* P1 = f2 + 1 < 9 ? T.Pcd : T.Pc;
         for (int t = tid; t < NN; t += THREADS) {
           const int i = t / N, m = t % N;
           const double h = m == 0 ? 0.5 : 1.0;
-          const double* P = (dth ? T.Pcd : T.Pc) + (i * N + m) * N;
-          double sa = 0.0, sb = 0.0;
+          const double* Q0 = P0 + (i * N + m) * N;
+          const double* Q1 = P1 + (i * N + m) * N;
+          double sa0 = 0.0, sb0 = 0.0, sa1 = 0.0, sb1 = 0.0;
           #pragma unroll 4
           for (int n = m; n < N; ++n) {
-            sa += P[n] * ca[(c * N + m) * N + n];
-            sb += P[n] * cb[(c * N + m) * N + n];
+            sa0 += Q0[n] * ca[(c0 * N + m) * N + n];
+            sb0 += Q0[n] * cb[(c0 * N + m) * N + n];
+            sa1 += Q1[n] * ca[(c1 * N + m) * N + n];
+            sb1 += Q1[n] * cb[(c1 * N + m) * N + n];
           }
-          tmp[2 * t] = h * sa, tmp[2 * t + 1] = h * sb;
+          tmp[2 * t] = h * sa0, tmp[2 * t + 1] = h * sb0;
+          tmp[2 * (NN + t)] = h * sa1, tmp[2 * (NN + t) + 1] = h * sb1;
         }
       }
       __syncthreads();
-      // the 2x-grid longitude DFT partials are dead once fi >= 6: x_theta / x_phi reuse them
-      double* dst = fi == 0 ? o0 : fi == 1 ? o1 : fi == 2 ? o2 : fi == 3 ? o3 : fi == 4 ? o4 : o5;
+      // the 2x-grid longitude DFT partials are dead once f2 >= 6: x_theta / x_phi reuse them
       for (int node = tid; node < nc; node += THREADS) {
         const int i = node / np2, j = node % np2;
-        double v = 0.0;
+        double v0 = 0.0, v1 = 0.0;
+        int q = 0;  // (m j) mod np2
         #pragma unroll 4
         for (int m = 0; m < N; ++m) {
-          const double A = tmp[2 * (i * N + m)], B = tmp[2 * (i * N + m) + 1];
-          const double cm = c2[j * N + m], sm = s2[j * N + m];
-          if (fi < 9)
-            v += A * cm - B * sm;
+          const double A0 = tmp[2 * (i * N + m)], B0 = tmp[2 * (i * N + m) + 1];
+          const double A1 = tmp[2 * (NN + i * N + m)], B1 = tmp[2 * (NN + i * N + m) + 1];
+          const double cm = tc2[q], sm = ts2[q];
+          q += j;
+          if (q >= np2) q -= np2;
+          if (f2 < 9)
+            v0 += A0 * cm - B0 * sm;
           else
-            v += (double)m * (-A * sm - B * cm);
+            v0 += (double)m * (-A0 * sm - B0 * cm);
+          if (f2 + 1 < 9)
+            v1 += A1 * cm - B1 * sm;
+          else
+            v1 += (double)m * (-A1 * sm - B1 * cm);
         }
-        if (fi < 6)
-          dst[(size_t)cap * nc + node] = v;
-        else if (fi < 9)
-          xt[(size_t)node * 3 + (fi - 6)] = v;
-        else
-          xp[(size_t)node * 3 + (fi - 9)] = v;
+        const double v[2] = {v0, v1};
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const int fi = f2 + e;
+          if (fi < 6) {
+            double* dst = fi == 0 ? o0 : fi == 1 ? o1 : fi == 2 ? o2 : fi == 3 ? o3 : fi == 4 ? o4 : o5;
+            dst[(size_t)cap * nc + node] = v[e];
+          } else if (fi < 9) {
+            xt[(size_t)node * 3 + (fi - 6)] = v[e];
+          } else {
+            xp[(size_t)node * 3 + (fi - 9)] = v[e];
+          }
+        }
       }
       __syncthreads();
     }
@@ -1156,8 +1223,9 @@ size_t geom_lds_bytes(const Tables& T) {
 size_t lds_bytes(const Tables& T) {
   const int N = T.N, NN = N * N;
   size_t work = std::max<size_t>(3 * NN * 2, (size_t)RING * 3 * 5 * N * 2 + (size_t)RING * T.np * 6);
-  work = std::max<size_t>(work, 8 * NN);  // phase 3: fc, tmp and the two [2N][N] tables
-  return (6 * NN + work) * sizeof(double);
+  // phase 3: the six fields' packed coefficients and the partial sums of two fields
+  work = std::max<size_t>(work, (size_t)6 * N * (N + 1) + 4 * NN);
+  return (6 * NN + 2 * (T.np + T.np2) + work) * sizeof(double);
 }
 
 template <int MODE>
@@ -1210,15 +1278,19 @@ int memb_plan_create(memb_plan** out, int L, int device) {
   }
   std::vector<double> cs2((size_t)np2 * N), sn2((size_t)np2 * N), csf((size_t)np * N),
       snf((size_t)np * N);
+  // cos / sin(m phi_j) with the angle reduced first, 2 pi ((m j) mod n) / n, as the longitude
+  // FFT's twiddle factors are formed (hrfft.f:214-267): exact to the last bit of cos / sin
+  // of an angle below 2 pi, where m phi_j itself (up to 2 pi L) carries 1e-14 of rounding
+  // that the fourth phi derivatives multiply by m^4
   for (int j = 0; j < np2; ++j)
     for (int m = 0; m < N; ++m) {
-      const double ph = 2.0 * M_PI * j / np2;
-      cs2[j * N + m] = std::cos(m * ph), sn2[j * N + m] = std::sin(m * ph);
+      const double ph = 2.0 * M_PI * ((m * j) % np2) / np2;
+      cs2[j * N + m] = std::cos(ph), sn2[j * N + m] = std::sin(ph);
     }
   for (int j = 0; j < np; ++j)
     for (int m = 0; m < N; ++m) {
-      const double ph = 2.0 * M_PI * j / np;
-      csf[j * N + m] = std::cos(m * ph), snf[j * N + m] = std::sin(m * ph);
+      const double ph = 2.0 * M_PI * ((m * j) % np) / np;
+      csf[j * N + m] = std::cos(ph), snf[j * N + m] = std::sin(ph);
     }
   if ((rc = upload(p, w2, &T.w2)) || (rc = upload(p, w, &T.w)) || (rc = upload(p, s2, &T.s2)) ||
       (rc = upload(p, Pc, &T.Pc)) || (rc = upload(p, Pcd, &T.Pcd)) ||

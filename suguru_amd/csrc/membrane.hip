@@ -539,19 +539,25 @@ __global__ void __launch_bounds__(THREADS, 2)
         // longitude DFT of the ring's six outputs, degrees <= L only.  A thread sums three
         // outputs for one (ring, m), so that each pair of table loads feeds six FMAs and the
         // 2 x 3 x 17 sums of a ring group fit one round of 128 threads.
+        // Nodes j and j + np/2 share their factors up to (-1)^m (phi -> phi + pi), so the
+        // sum runs over the first half of the ring on v_j + (-1)^m v_{j + np/2}.
         for (int t = tid; t < nr * 2 * N; t += THREADS) {
           const int r = t / (2 * N), f0 = 3 * ((t / N) % 2), m = t % N;
+          const double sg = (m & 1) ? -1.0 : 1.0;
+          const int hp = np / 2;
           double cs0 = 0.0, sn0 = 0.0, cs1 = 0.0, sn1 = 0.0, cs2 = 0.0, sn2 = 0.0;
           int q = 0;  // (m j) mod np
           #pragma unroll 4
-          for (int j = 0; j < np; ++j) {
+          for (int j = 0; j < hp; ++j) {
             const double* v = F + ((size_t)r * np + j) * 6 + f0;
+            const double* w = v + (size_t)hp * 6;
             const double c = tcf[q], s = tsf[q];
             q += m;
             if (q >= np) q -= np;
-            cs0 += v[0] * c, sn0 += v[0] * s;
-            cs1 += v[1] * c, sn1 += v[1] * s;
-            cs2 += v[2] * c, sn2 += v[2] * s;
+            const double e0 = v[0] + sg * w[0], e1 = v[1] + sg * w[1], e2 = v[2] + sg * w[2];
+            cs0 += e0 * c, sn0 += e0 * s;
+            cs1 += e1 * c, sn1 += e1 * s;
+            cs2 += e2 * c, sn2 += e2 * s;
           }
           double* s = scr + (((size_t)(i0 + r) * 6 + f0) * N + m) * 2;
           s[0] = cs0, s[1] = sn0;
@@ -573,28 +579,52 @@ __global__ void __launch_bounds__(THREADS, 2)
     double* xt = scr;            // [nc][3] x_theta on the force grid (scratch reused)
     double* xp = scr + (size_t)nc * 3;  // [nc][3] x_phi
     if (MODE == 0) {
-      // quadrature of all six fields in one pass, one thread per (m, n >= m): each table load
-      // feeds twelve sums
-      for (int t = tid; t < NT; t += THREADS) {
-        int m = 0, n = t;
-        while (n >= N - m) n -= N - m, ++m;
-        n += m;
-        double sa[6], sb[6];
+      // quadrature of all six fields in one pass, one item per (m, n >= m) pair (each table
+      // load feeds twelve sums), 2 THREADS items per round.  The scratch rows of nq rings at
+      // a time are first copied into the tmp region (all of a thread's loads in flight, one
+      // wait per chunk): read in place, each iteration waited on its own scratch loads.
+      const int nq = N / 3;  // 12 N nq <= 4 NN doubles
+      double* stg = tmp;
+      for (int t0 = 0; t0 < NT; t0 += 2 * THREADS) {
+        int mm[2], nn[2];
+        double sa[2][6], sb[2][6];
 #pragma unroll
-        for (int f = 0; f < 6; ++f) sa[f] = 0.0, sb[f] = 0.0;
-        #pragma unroll 2
-        for (int i = 0; i < nt; ++i) {
-          const double p = T.w[i] * T.Pfa[((size_t)i * N + m) * N + n];
-          const double* s = scr + (((size_t)i * 6) * N + m) * 2;
+        for (int k = 0; k < 2; ++k) {
+          int m = 0, n = min(t0 + k * THREADS + tid, NT - 1);
+          while (n >= N - m) n -= N - m, ++m;
+          mm[k] = m, nn[k] = n + m;
 #pragma unroll
-          for (int f = 0; f < 6; ++f) {
-            sa[f] += p * s[2 * f * N];
-            sb[f] += p * s[2 * f * N + 1];
+          for (int f = 0; f < 6; ++f) sa[k][f] = 0.0, sb[k][f] = 0.0;
+        }
+        for (int c0 = 0; c0 < nt; c0 += nq) {
+          const int cr = min(nq, nt - c0);
+          __syncthreads();  // the previous chunk is consumed
+          const double2* src = reinterpret_cast<const double2*>(scr + (size_t)c0 * 12 * N);
+          double2* dst = reinterpret_cast<double2*>(stg);
+          for (int e = tid; e < cr * 6 * N; e += THREADS) dst[e] = src[e];
+          __syncthreads();
+          for (int r = 0; r < cr; ++r) {
+            const int i = c0 + r;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+              const double p = T.w[i] * T.Pfa[((size_t)i * N + mm[k]) * N + nn[k]];
+              const double* sr = stg + ((size_t)r * 6 * N + mm[k]) * 2;
+#pragma unroll
+              for (int f = 0; f < 6; ++f) {
+                sa[k][f] += p * sr[2 * f * N];
+                sb[k][f] += p * sr[2 * f * N + 1];
+              }
+            }
           }
         }
 #pragma unroll
-        for (int f = 0; f < 6; ++f)
-          fc[2 * (f * NT + t)] = 2.0 / np * sa[f], fc[2 * (f * NT + t) + 1] = -2.0 / np * sb[f];
+        for (int k = 0; k < 2; ++k) {
+          const int t = t0 + k * THREADS + tid;
+          if (t < NT)
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+              fc[2 * (f * NT + t)] = 2.0 / np * sa[k][f], fc[2 * (f * NT + t) + 1] = -2.0 / np * sb[k][f];
+        }
       }
       __syncthreads();
     }
@@ -642,31 +672,31 @@ __global__ void __launch_bounds__(THREADS, 2)
         }
       }
       __syncthreads();
-      // the 2x-grid longitude DFT partials are dead once f2 >= 6: x_theta / x_phi reuse them
-      for (int node = tid; node < nc; node += THREADS) {
-        const int i = node / np2, j = node % np2;
-        double v0 = 0.0, v1 = 0.0;
+      // the 2x-grid longitude DFT partials are dead once f2 >= 6: x_theta / x_phi reuse them.
+      // One thread per node (i, j < np2/2) and its partner j + np2/2, whose factors are the
+      // same up to (-1)^m (phi -> phi + pi).
+      const int hp2 = np2 / 2;
+      for (int t = tid; t < N * hp2; t += THREADS) {
+        const int i = t / hp2, j = t % hp2;
+        double v0 = 0.0, v1 = 0.0, w0 = 0.0, w1 = 0.0;  // v: node j, w: node j + np2/2
         int q = 0;  // (m j) mod np2
         #pragma unroll 4
         for (int m = 0; m < N; ++m) {
           const double A0 = tmp[2 * (i * N + m)], B0 = tmp[2 * (i * N + m) + 1];
           const double A1 = tmp[2 * (NN + i * N + m)], B1 = tmp[2 * (NN + i * N + m) + 1];
-          const double cm = tc2[q], sm = ts2[q];
+          const double cm = tc2[q], sm = ts2[q], sg = (m & 1) ? -1.0 : 1.0;
           q += j;
           if (q >= np2) q -= np2;
-          if (f2 < 9)
-            v0 += A0 * cm - B0 * sm;
-          else
-            v0 += (double)m * (-A0 * sm - B0 * cm);
-          if (f2 + 1 < 9)
-            v1 += A1 * cm - B1 * sm;
-          else
-            v1 += (double)m * (-A1 * sm - B1 * cm);
+          const double t0 = f2 < 9 ? A0 * cm - B0 * sm : (double)m * (-A0 * sm - B0 * cm);
+          const double t1 = f2 + 1 < 9 ? A1 * cm - B1 * sm : (double)m * (-A1 * sm - B1 * cm);
+          v0 += t0, w0 += sg * t0;
+          v1 += t1, w1 += sg * t1;
         }
-        const double v[2] = {v0, v1};
+        const double v[4] = {v0, v1, w0, w1};
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          const int fi = f2 + e;
+        for (int e = 0; e < 4; ++e) {
+          const int fi = f2 + (e & 1);
+          const int node = i * np2 + j + (e >> 1) * hp2;
           if (fi < 6) {
             double* dst = fi == 0 ? o0 : fi == 1 ? o1 : fi == 2 ? o2 : fi == 3 ? o3 : fi == 4 ? o4 : o5;
             dst[(size_t)cap * nc + node] = v[e];

@@ -463,15 +463,27 @@ __global__ void __launch_bounds__(THREADS, 2)
       __syncthreads();
       continue;
     }
-    // 2. the 2x grid, RING latitudes at a time (not for MODE 3: geometry only)
+    // 2. the 2x grid in chunks of W consecutive nodes, one node per thread (not for MODE 3:
+    //    geometry only).  A chunk of W <= (RING - 1) np + 1 nodes touches at most RING rings.
+    //    Ring r's coefficients and outputs live in slot r % RING of AB and F: its coefficients
+    //    are summed before the first chunk that touches it, its longitude DFT runs after the
+    //    chunk that completes it, and only ring r + RING takes the slot again, whose first node
+    //    lies (RING - 1) np + 1 >= W nodes after ring r's last (so in a later chunk).  With
+    //    W = THREADS every round of the per-node phase is full (one partial round per capsule
+    //    instead of one per group of RING rings).
     double* AB = U;                          // [RING][3][5][N][2]
     double* F = AB + RING * 3 * 5 * N * 2;   // [RING][np][6]
-    for (int i0 = 0; i0 < (MODE == 3 ? 0 : nt); i0 += RING) {
-      const int nr = (nt - i0) < RING ? (nt - i0) : RING;
+    const int nnode = MODE == 3 ? 0 : nt * np;
+    const int W = min(THREADS, (RING - 1) * np + 1);
+    int rnext = 0, rdone = 0;  // first ring without coefficients / without its DFT
+    for (int n0 = 0; n0 < nnode; n0 += W) {
+      const int n1 = min(n0 + W, nnode);
+      const int rhi = (n1 - 1) / np;  // last ring of the chunk
       // a thread sums the three coordinates of one (ring, k, m): each table load feeds six FMAs
-      for (int t = tid; t < nr * 5 * N; t += THREADS) {
-        const int r = t / (5 * N), k = (t / N) % 5, m = t % N;
-        const double* Pd = T.Pfd + (((size_t)(i0 + r) * 5 + k) * N + m) * N;
+      for (int t = tid; t < (rhi + 1 - rnext) * 5 * N; t += THREADS) {
+        const int ring = rnext + t / (5 * N), k = (t / N) % 5, m = t % N;
+        const int r = ring % RING;
+        const double* Pd = T.Pfd + (((size_t)ring * 5 + k) * N + m) * N;
         const double h = m == 0 ? 0.5 : 1.0;
         double sa0 = 0.0, sb0 = 0.0, sa1 = 0.0, sb1 = 0.0, sa2 = 0.0, sb2 = 0.0;
         #pragma unroll 4
@@ -486,9 +498,10 @@ __global__ void __launch_bounds__(THREADS, 2)
         AB[2 * (t0 + cs)] = h * sa1, AB[2 * (t0 + cs) + 1] = h * sb1;
         AB[2 * (t0 + 2 * cs)] = h * sa2, AB[2 * (t0 + 2 * cs) + 1] = h * sb2;
       }
+      rnext = rhi + 1;
       __syncthreads();
-      for (int t = tid; t < nr * np; t += THREADS) {
-        const int r = t / np, j = t % np;
+      for (int node = n0 + tid; node < n1; node += THREADS) {
+        const int ring = node / np, j = node % np, r = ring % RING;
         double X[3][NQ];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -516,7 +529,6 @@ __global__ void __launch_bounds__(THREADS, 2)
         }
         Geom o;
         geometry(X, o);
-        const size_t node = (size_t)(i0 + r) * np + j;
         if (MODE == 1) {
           double* R = o0 + node * NREF;
 #pragma unroll
@@ -529,20 +541,21 @@ __global__ void __launch_bounds__(THREADS, 2)
         } else {
           double f[3];
           law(o, T.ref + node * NREF, gs, eb, f);
-          double* Fo = F + (size_t)t * 6;
+          double* Fo = F + ((size_t)r * np + j) * 6;
           Fo[0] = f[0], Fo[1] = f[1], Fo[2] = f[2];
           Fo[3] = o.n[0], Fo[4] = o.n[1], Fo[5] = o.n[2];
         }
       }
       __syncthreads();
-      if (MODE == 0) {
-        // longitude DFT of the ring's six outputs, degrees <= L only.  A thread sums three
-        // outputs for one (ring, m), so that each pair of table loads feeds six FMAs and the
-        // 2 x 3 x 17 sums of a ring group fit one round of 128 threads.
+      if (MODE == 0 && n1 / np > rdone) {
+        // rings rdone .. n1 / np - 1 are complete: longitude DFT of their six outputs, degrees
+        // <= L only.  A thread sums three outputs for one (ring, m), so that each pair of
+        // table loads feeds six FMAs.
         // Nodes j and j + np/2 share their factors up to (-1)^m (phi -> phi + pi), so the
         // sum runs over the first half of the ring on v_j + (-1)^m v_{j + np/2}.
-        for (int t = tid; t < nr * 2 * N; t += THREADS) {
-          const int r = t / (2 * N), f0 = 3 * ((t / N) % 2), m = t % N;
+        for (int t = tid; t < (n1 / np - rdone) * 2 * N; t += THREADS) {
+          const int ring = rdone + t / (2 * N), f0 = 3 * ((t / N) % 2), m = t % N;
+          const int r = ring % RING;
           const double sg = (m & 1) ? -1.0 : 1.0;
           const int hp = np / 2;
           double cs0 = 0.0, sn0 = 0.0, cs1 = 0.0, sn1 = 0.0, cs2 = 0.0, sn2 = 0.0;
@@ -559,12 +572,13 @@ __global__ void __launch_bounds__(THREADS, 2)
             cs1 += e1 * c, sn1 += e1 * s;
             cs2 += e2 * c, sn2 += e2 * s;
           }
-          double* s = scr + (((size_t)(i0 + r) * 6 + f0) * N + m) * 2;
+          double* s = scr + (((size_t)ring * 6 + f0) * N + m) * 2;
           s[0] = cs0, s[1] = sn0;
           s[2 * N] = cs1, s[2 * N + 1] = sn1;
           s[4 * N] = cs2, s[4 * N + 1] = sn2;
         }
         __syncthreads();
+        rdone = n1 / np;
       }
     }
     if (MODE == 1) continue;

@@ -1168,15 +1168,17 @@ __global__ void __launch_bounds__(GM_THREADS)
            double* __restrict__ nz, double* __restrict__ area, const int* __restrict__ skip) {
   extern __shared__ double lds[];
   const int tid = threadIdx.x;
-  const int N = T.N, NN = N * N, np2 = T.np2, nc = N * np2;
-  double* c2 = lds;                 // [np2][N]
-  double* s2 = c2 + np2 * N;        // [np2][N]
-  double* U = s2 + np2 * N;         // [3][N][N][2] longitude DFT
-  double* ca = U + 6 * NN;          // [3][N][N]
+  const int N = T.N, NN = N * N, np2 = T.np2, nc = N * np2, NT = N * (N + 1) / 2, hp = np2 / 2;
+  // cos / sin(2 pi q / np2), q < np2: the factor cos / sin(m phi_j) is entry (m j) mod np2, and
+  // points j and j + np2/2 share their factors up to (-1)^m (phi -> phi + pi)
+  double* tc = lds;
+  double* ts = tc + np2;
+  double* ca = ts + np2;            // [3][N][N]
   double* cb = ca + 3 * NN;         // [3][N][N]
   double* tmp = cb + 3 * NN;        // [6][N][N][2]: x_theta (0..2), x_phi (3..5) partial sums
-  double* pts = tmp;                // [3][nc] the capsule's points (dead before tmp is written)
-  for (int t = tid; t < np2 * N; t += GM_THREADS) c2[t] = T.cs2[t], s2[t] = T.sn2[t];
+  double* pts = tmp;                // [3][nc] the capsule's points and
+  double* U = tmp + 3 * nc;         // [3][N][N][2] their longitude DFT, both dead before tmp
+  for (int q = tid; q < np2; q += GM_THREADS) tc[q] = T.cs2[q * N + 1], ts[q] = T.sn2[q * N + 1];
   for (long long cap = blockIdx.x; cap < ncap; cap += gridDim.x) {
     if (skip && skip[cap]) continue;  // uniform over the workgroup
     // the capsule's points into LDS with coalesced loads
@@ -1185,75 +1187,100 @@ __global__ void __launch_bounds__(GM_THREADS)
       pts[t] = (c == 0 ? x : (c == 1 ? y : z))[cap * nc + q];
     }
     __syncthreads();
-    for (int t = tid; t < 3 * NN; t += GM_THREADS) {
-      const int c = t / NN, i = (t / N) % N, m = t % N;
-      const double* f = pts + c * nc + i * np2;
-      double cs = 0.0, sn = 0.0;
+    // longitude DFT, one thread per (i, m) for the three coordinates
+    for (int t = tid; t < NN; t += GM_THREADS) {
+      const int i = t / N, m = t % N;
+      const double sg = (m & 1) ? -1.0 : 1.0;
+      double cs[3] = {0.0, 0.0, 0.0}, sn[3] = {0.0, 0.0, 0.0};
+      int q = 0;  // (m j) mod np2
       #pragma unroll 4
-      for (int j = 0; j < np2; ++j) {
-        const double v = f[j];
-        cs += v * c2[j * N + m];
-        sn += v * s2[j * N + m];
-      }
-      U[2 * t] = cs, U[2 * t + 1] = sn;
-    }
-    __syncthreads();
-    for (int t = tid; t < 3 * NN; t += GM_THREADS) {
-      const int c = t / NN, m = (t / N) % N, n = t % N;
-      double sa = 0.0, sb = 0.0;
-      if (n >= m)
-        #pragma unroll 4
-        for (int i = 0; i < N; ++i) {
-          const double p = T.w2[i] * T.Pc[(i * N + m) * N + n];
-          const int u = (c * N + i) * N + m;
-          sa += p * U[2 * u];
-          sb += p * U[2 * u + 1];
+      for (int j = 0; j < hp; ++j) {
+        const double c = tc[q], s = ts[q];
+        q += m;
+        if (q >= np2) q -= np2;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const double* f = pts + k * nc + i * np2 + j;
+          const double v = f[0] + sg * f[hp];
+          cs[k] += v * c;
+          sn[k] += v * s;
         }
-      ca[(c * N + m) * N + n] = 2.0 / np2 * sa;
-      cb[(c * N + m) * N + n] = -2.0 / np2 * sb;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) U[2 * (k * NN + t)] = cs[k], U[2 * (k * NN + t) + 1] = sn[k];
     }
     __syncthreads();
-    for (int t = tid; t < 6 * NN; t += GM_THREADS) {
-      const int f = t / NN, i = (t / N) % N, m = t % N;
-      const bool dth = f < 3;
-      const int c = dth ? f : f - 3;
+    // quadrature, one thread per (m, n >= m) for the three coordinates (n < m is never read)
+    for (int t = tid; t < NT; t += GM_THREADS) {
+      int m = 0, n = t;
+      while (n >= N - m) n -= N - m, ++m;
+      n += m;
+      double sa[3] = {0.0, 0.0, 0.0}, sb[3] = {0.0, 0.0, 0.0};
+      #pragma unroll 4
+      for (int i = 0; i < N; ++i) {
+        const double p = T.w2[i] * T.Pc[(i * N + m) * N + n];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int u = (k * N + i) * N + m;
+          sa[k] += p * U[2 * u];
+          sb[k] += p * U[2 * u + 1];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        ca[(k * N + m) * N + n] = 2.0 / np2 * sa[k];
+        cb[(k * N + m) * N + n] = -2.0 / np2 * sb[k];
+      }
+    }
+    __syncthreads();
+    // x_theta and x_phi of coordinate c per (c, i, m): both from the same coefficients
+    for (int t = tid; t < 3 * NN; t += GM_THREADS) {
+      const int c = t / NN, u = t % NN, i = u / N, m = u % N;
       const double h = m == 0 ? 0.5 : 1.0;
-      const double* P = (dth ? T.Pcd : T.Pc) + (i * N + m) * N;
-      double sa = 0.0, sb = 0.0;
+      const double* Pt = T.Pcd + (i * N + m) * N;
+      const double* Pp = T.Pc + (i * N + m) * N;
+      double sa0 = 0.0, sb0 = 0.0, sa1 = 0.0, sb1 = 0.0;
       #pragma unroll 4
       for (int n = m; n < N; ++n) {
-        sa += P[n] * ca[(c * N + m) * N + n];
-        sb += P[n] * cb[(c * N + m) * N + n];
+        const double a = ca[(c * N + m) * N + n], b = cb[(c * N + m) * N + n];
+        sa0 += Pt[n] * a;
+        sb0 += Pt[n] * b;
+        sa1 += Pp[n] * a;
+        sb1 += Pp[n] * b;
       }
-      tmp[2 * t] = h * sa, tmp[2 * t + 1] = h * sb;
+      tmp[2 * t] = h * sa0, tmp[2 * t + 1] = h * sb0;
+      tmp[2 * (3 * NN + t)] = h * sa1, tmp[2 * (3 * NN + t) + 1] = h * sb1;
     }
     __syncthreads();
-    for (int node = tid; node < nc; node += GM_THREADS) {
-      const int i = node / np2, j = node % np2;
-      double xt[3], xp[3];
-      for (int f = 0; f < 6; ++f) {
-        const double* tf = tmp + 2 * (f * NN + i * N);
-        double v = 0.0;
-        #pragma unroll 4
-        for (int m = 0; m < N; ++m) {
-          const double A = tf[2 * m], B = tf[2 * m + 1];
-          const double cm = c2[j * N + m], sm = s2[j * N + m];
-          if (f < 3)
-            v += A * cm - B * sm;
-          else
-            v += (double)m * (-A * sm - B * cm);
+    // node sums, one thread per (i, j < np2/2) and its partner j + np2/2; normal and area
+    for (int t = tid; t < N * hp; t += GM_THREADS) {
+      const int i = t / hp, j = t % hp;
+      double d[2][6];
+#pragma unroll
+      for (int f = 0; f < 6; ++f) d[0][f] = 0.0, d[1][f] = 0.0;
+      int q = 0;  // (m j) mod np2
+      #pragma unroll 2
+      for (int m = 0; m < N; ++m) {
+        const double cm = tc[q], sm = ts[q], sg = (m & 1) ? -1.0 : 1.0;
+        q += j;
+        if (q >= np2) q -= np2;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+          const double A = tmp[2 * (f * NN + i * N + m)], B = tmp[2 * (f * NN + i * N + m) + 1];
+          const double e = f < 3 ? A * cm - B * sm : (double)m * (-A * sm - B * cm);
+          d[0][f] += e;
+          d[1][f] += sg * e;
         }
-        if (f < 3)
-          xt[f] = v;
-        else
-          xp[f - 3] = v;
       }
-      double cr[3];
-      cross3(xt, xp, cr);
-      const double J = sqrt(dot3(cr, cr));
-      const size_t o = (size_t)cap * nc + node;
-      nx[o] = cr[0] / J, ny[o] = cr[1] / J, nz[o] = cr[2] / J;
-      area[o] = J * T.w2[i] / T.s2[i] * (2.0 * M_PI / np2);
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        double cr[3];
+        cross3(&d[e][0], &d[e][3], cr);
+        const double J = sqrt(dot3(cr, cr));
+        const size_t o = (size_t)cap * nc + i * np2 + j + e * hp;
+        nx[o] = cr[0] / J, ny[o] = cr[1] / J, nz[o] = cr[2] / J;
+        area[o] = J * T.w2[i] / T.s2[i] * (2.0 * M_PI / np2);
+      }
     }
     __syncthreads();
   }
@@ -1261,7 +1288,9 @@ __global__ void __launch_bounds__(GM_THREADS)
 
 size_t geom_lds_bytes(const Tables& T) {
   const int N = T.N, NN = N * N;
-  return (size_t)(2 * T.np2 * N + 6 * NN + 6 * NN + 12 * NN) * sizeof(double);
+  // the two longitude tables, the coefficients, and the partial sums (which hold the points
+  // and their longitude DFT before)
+  return (size_t)(2 * T.np2 + 6 * NN + std::max(12 * NN, 3 * N * T.np2 + 6 * NN)) * sizeof(double);
 }
 
 size_t lds_bytes(const Tables& T) {
@@ -1450,8 +1479,8 @@ int memb_geometry_f64_dev(memb_plan* p, int64_t ncap, const double* x, const dou
     return fail(p, MEMB_E_ARG, "null pointer");
   MCHK(p, hipSetDevice(p->device));
   if (ncap == 0) return MEMB_OK;
-  // two 256-thread workgroups per CU (LDS: 2 x 65 KB at L = 16)
-  const unsigned nwg = (unsigned)std::min<long long>(ncap, (long long)p->num_wg / 2);
+  // three 256-thread workgroups per CU (LDS: 3 x 42 KB at L = 16)
+  const unsigned nwg = (unsigned)std::min<long long>(ncap, (long long)p->num_wg * 3 / 4);
   k_geom<<<nwg, GM_THREADS, geom_lds_bytes(p->T), p->stream>>>(p->T, ncap, x, y, z, nx, ny, nz,
                                                                area, skip);
   MCHK(p, hipGetLastError());

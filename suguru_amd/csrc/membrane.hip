@@ -409,20 +409,28 @@ __global__ void __launch_bounds__(THREADS, 2)
     if (sel && sel[cap] == 0) continue;  // (uniform in the workgroup)
     const double* xyz[3] = {x + cap * nc, y + cap * nc, z + cap * nc};
     // 1. analysis: longitude DFT per (c, i, m) into U, then quadrature per (c, m, n)
-    for (int t = tid; t < 3 * NN; t += THREADS) {
-      const int c = t / NN, i = (t / N) % N, m = t % N;
-      const double* f = xyz[c] + i * np2;
-      double cs = 0.0, sn = 0.0;
+    // one thread per (i, m) for the three coordinates; points j and j + np2/2 share their
+    // factors up to (-1)^m (phi -> phi + pi)
+    for (int t = tid; t < NN; t += THREADS) {
+      const int i = t / N, m = t % N, hp2 = np2 / 2;
+      const double sg = (m & 1) ? -1.0 : 1.0;
+      double cs[3] = {0.0, 0.0, 0.0}, sn[3] = {0.0, 0.0, 0.0};
       int q = 0;  // (m j) mod np2
       #pragma unroll 4
-      for (int j = 0; j < np2; ++j) {
-        const double v = f[j];
-        cs += v * tc2[q];
-        sn += v * ts2[q];
+      for (int j = 0; j < hp2; ++j) {
+        const double c = tc2[q], s = ts2[q];
         q += m;
         if (q >= np2) q -= np2;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const double* f = xyz[k] + i * np2 + j;
+          const double v = f[0] + sg * f[hp2];
+          cs[k] += v * c;
+          sn[k] += v * s;
+        }
       }
-      U[2 * t] = cs, U[2 * t + 1] = sn;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) U[2 * (k * NN + t)] = cs[k], U[2 * (k * NN + t) + 1] = sn[k];
     }
     __syncthreads();
     // one thread per (m, n >= m) for the three coordinates (entries n < m are never read)

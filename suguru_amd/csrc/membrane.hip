@@ -409,22 +409,23 @@ __global__ void __launch_bounds__(THREADS, 2)
     if (sel && sel[cap] == 0) continue;  // (uniform in the workgroup)
     const double* xyz[3] = {x + cap * nc, y + cap * nc, z + cap * nc};
     // 1. analysis: longitude DFT per (c, i, m) into U, then quadrature per (c, m, n)
-    // one thread per (i, m) for the three coordinates; points j and j + np2/2 share their
-    // factors up to (-1)^m (phi -> phi + pi)
+    // one thread per (i, m) for the three coordinates, every point in order: the raw
+    // coordinates carry the capsule's position, and pairing points j and j + np2/2 (the
+    // phi -> phi + pi symmetry used for the outputs below) rounds the near-zero high-m
+    // coefficients differently enough for the fourth derivatives' m^4 to show (4x the
+    // difference to the oracle on an ellipsoid)
     for (int t = tid; t < NN; t += THREADS) {
-      const int i = t / N, m = t % N, hp2 = np2 / 2;
-      const double sg = (m & 1) ? -1.0 : 1.0;
+      const int i = t / N, m = t % N;
       double cs[3] = {0.0, 0.0, 0.0}, sn[3] = {0.0, 0.0, 0.0};
       int q = 0;  // (m j) mod np2
       #pragma unroll 4
-      for (int j = 0; j < hp2; ++j) {
+      for (int j = 0; j < np2; ++j) {
         const double c = tc2[q], s = ts2[q];
         q += m;
         if (q >= np2) q -= np2;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          const double* f = xyz[k] + i * np2 + j;
-          const double v = f[0] + sg * f[hp2];
+          const double v = xyz[k][i * np2 + j];
           cs[k] += v * c;
           sn[k] += v * s;
         }
@@ -472,7 +473,7 @@ __global__ void __launch_bounds__(THREADS, 2)
       continue;
     }
     // 2. the 2x grid in chunks of W consecutive nodes, one node per thread (not for MODE 3:
-    //    geometry only).  A chunk of W <= (RING - 1) np + 1 nodes touches at most RING rings.
+    //    geometry only).  A chunk of W <= (RING - 1) np nodes touches at most RING rings.
     //    Ring r's coefficients and outputs live in slot r % RING of AB and F: its coefficients
     //    are summed before the first chunk that touches it, its longitude DFT runs after the
     //    chunk that completes it, and only ring r + RING takes the slot again, whose first node
@@ -482,7 +483,7 @@ __global__ void __launch_bounds__(THREADS, 2)
     double* AB = U;                          // [RING][3][5][N][2]
     double* F = AB + RING * 3 * 5 * N * 2;   // [RING][np][6]
     const int nnode = MODE == 3 ? 0 : nt * np;
-    const int W = min(THREADS, (RING - 1) * np + 1);
+    const int W = min(THREADS, (RING - 1) * np);  // even: node pairs stay in one chunk
     int rnext = 0, rdone = 0;  // first ring without coefficients / without its DFT
     for (int n0 = 0; n0 < nnode; n0 += W) {
       const int n1 = min(n0 + W, nnode);
@@ -508,17 +509,24 @@ __global__ void __launch_bounds__(THREADS, 2)
       }
       rnext = rhi + 1;
       __syncthreads();
-      for (int node = n0 + tid; node < n1; node += THREADS) {
-        const int ring = node / np, j = node % np, r = ring % RING;
+      // Lanes 2k and 2k + 1 take the node pair (jj, jj + np/2) of one ring, whose factors
+      // agree up to (-1)^m (phi -> phi + pi): lane 2k sums the even-m terms E and lane 2k + 1
+      // the odd-m terms O, both at phi_jj; after one exchange node jj holds E + O and node
+      // jj + np/2 holds E - O, and each lane goes on with its own node.
+      for (int pn = n0 + tid; pn < n1; pn += THREADS) {
+        const int par = pn & 1, pr = pn >> 1, hp = np / 2;
+        const int ring = pr / hp, jj = pr - ring * hp, j = jj + par * hp, r = ring % RING;
+        const int node = ring * np + j;
         double X[3][NQ];
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
           for (int q = 0; q < NQ; ++q) X[c][q] = 0.0;
-        int q = 0;  // (m j) mod np
-        for (int m = 0; m < N; ++m) {
+        int q = par * jj;  // (m jj) mod np
+        const int dq = (2 * jj) % np;
+        for (int m = par; m < N; m += 2) {
           const double cm = tcf[q], sm = tsf[q];
-          q += j;
+          q += dq;
           if (q >= np) q -= np;
           const double m1 = (double)m, m2 = m1 * m1, m3 = m2 * m1, m4 = m2 * m2;
 #pragma unroll
@@ -535,6 +543,13 @@ __global__ void __launch_bounds__(THREADS, 2)
               if (kt == 0) X[c][qidx(0, 4)] += m4 * h0;
             }
         }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int qq = 0; qq < NQ; ++qq) {
+            const double other = __shfl_xor(X[c][qq], 1);
+            X[c][qq] = par ? other - X[c][qq] : X[c][qq] + other;
+          }
         Geom o;
         geometry(X, o);
         if (MODE == 1) {
@@ -1178,7 +1193,7 @@ __global__ void __launch_bounds__(GM_THREADS)
   const int tid = threadIdx.x;
   const int N = T.N, NN = N * N, np2 = T.np2, nc = N * np2, NT = N * (N + 1) / 2, hp = np2 / 2;
   // cos / sin(2 pi q / np2), q < np2: the factor cos / sin(m phi_j) is entry (m j) mod np2, and
-  // points j and j + np2/2 share their factors up to (-1)^m (phi -> phi + pi)
+  // nodes j and j + np2/2 share their factors up to (-1)^m (phi -> phi + pi)
   double* tc = lds;
   double* ts = tc + np2;
   double* ca = ts + np2;            // [3][N][N]
@@ -1195,21 +1210,20 @@ __global__ void __launch_bounds__(GM_THREADS)
       pts[t] = (c == 0 ? x : (c == 1 ? y : z))[cap * nc + q];
     }
     __syncthreads();
-    // longitude DFT, one thread per (i, m) for the three coordinates
+    // longitude DFT, one thread per (i, m) for the three coordinates, every point in order
+    // (the raw coordinates are not paired by the phi -> phi + pi symmetry, as in the law)
     for (int t = tid; t < NN; t += GM_THREADS) {
       const int i = t / N, m = t % N;
-      const double sg = (m & 1) ? -1.0 : 1.0;
       double cs[3] = {0.0, 0.0, 0.0}, sn[3] = {0.0, 0.0, 0.0};
       int q = 0;  // (m j) mod np2
       #pragma unroll 4
-      for (int j = 0; j < hp; ++j) {
+      for (int j = 0; j < np2; ++j) {
         const double c = tc[q], s = ts[q];
         q += m;
         if (q >= np2) q -= np2;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-          const double* f = pts + k * nc + i * np2 + j;
-          const double v = f[0] + sg * f[hp];
+          const double v = pts[k * nc + i * np2 + j];
           cs[k] += v * c;
           sn[k] += v * s;
         }

@@ -56,7 +56,7 @@ def capsules(L, ncap, seed, amp=0.15):
     return np.concatenate(xs), np.concatenate(ys), np.concatenate(zs)
 
 
-@pytest.mark.parametrize("L,eb", [(16, 0.3), (16, 0.0), (8, 0.3), (23, 0.3)])
+@pytest.mark.parametrize("L,eb", [(16, 0.3), (16, 0.0), (8, 0.3), (4, 0.3), (23, 0.3)])
 def test_membrane_law_matches_oracle(L, eb):
     torch = _torch()
     from suguru_amd.membrane import MembranePlan

@@ -45,8 +45,8 @@ class CoupledStep:
     overlap: run the interpolation on a second stream beside the traction stage in the graphs
     and in untimed eager steps (steps timed with events stay sequential).  None (default):
     only with a traction object -- beside the FP64 law, whose persistent workgroups then
-    share the compute units, the step is slower (37.3 against 36.5 ms at 8000 capsules); beside
-    the surrogate stage it saves 0.65 ms.
+    share the compute units, the step was slower (37.3 against 36.5 ms at 8000 capsules); beside
+    the surrogate stage it saves 0.17 ms (25.15 against 25.33 ms).
 
     grid: ibx-style grid (itot, jtot powers of two); cap_off: every capsule has the points of
     degree `degree`, latitude-major; gs, eb: the law's moduli; rep, dt: param.f90:57 and

@@ -103,8 +103,9 @@ __device__ void geometry(const double (*X)[NQ], Geom& o) {
   double cv[3], nd[2][3];
   cross3(x1[0], x1[1], cv);
   o.J = sqrt(dot3(cv, cv));
+  const double rJ = 1.0 / o.J;  // one division per node; the others multiply by it
 #pragma unroll
-  for (int k = 0; k < 3; ++k) o.n[k] = cv[k] / o.J;
+  for (int k = 0; k < 3; ++k) o.n[k] = cv[k] * rJ;
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
     double t1[3], t2[3], cd[3];
@@ -114,7 +115,7 @@ __device__ void geometry(const double (*X)[NQ], Geom& o) {
     for (int k = 0; k < 3; ++k) cd[k] = t1[k] + t2[k];
     const double Jd = dot3(o.n, cd);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) nd[c][k] = (cd[k] - Jd * o.n[k]) / o.J;
+    for (int k = 0; k < 3; ++k) nd[c][k] = (cd[k] - Jd * o.n[k]) * rJ;
   }
   double gdd[2][2][2][2];
 #pragma unroll
@@ -132,8 +133,9 @@ __device__ void geometry(const double (*X)[NQ], Geom& o) {
       }
     }
   const double det = o.g[0][0] * o.g[1][1] - o.g[0][1] * o.g[1][0];
-  o.G[0][0] = o.g[1][1] / det, o.G[1][1] = o.g[0][0] / det;
-  o.G[0][1] = -o.g[0][1] / det, o.G[1][0] = -o.g[1][0] / det;
+  const double rdet = 1.0 / det;
+  o.G[0][0] = o.g[1][1] * rdet, o.G[1][1] = o.g[0][0] * rdet;
+  o.G[0][1] = -o.g[0][1] * rdet, o.G[1][0] = -o.g[1][0] * rdet;
   double GdgG[2][2][2];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
@@ -233,6 +235,7 @@ __device__ void law(const Geom& o, const double* __restrict__ R, double gs, doub
   const double deta = o.g[0][0] * o.g[1][1] - o.g[0][1] * o.g[0][1];
   const double I2 = detA * deta - 1.0;
   const double Js = sqrt(1.0 + I2);
+  const double rJs = 1.0 / Js, rI = 1.0 / (I2 + 1.0);  // one division each
   double I2d[2], Jsd[2];
 #pragma unroll
   for (int c = 0; c < 2; ++c) {
@@ -240,22 +243,22 @@ __device__ void law(const Geom& o, const double* __restrict__ R, double gs, doub
     const double da = o.gd[c][0][0] * o.g[1][1] + o.g[0][0] * o.gd[c][1][1] -
                       2.0 * o.g[0][1] * o.gd[c][0][1];
     I2d[c] = dA * deta + da * detA;
-    Jsd[c] = 0.5 / Js * I2d[c];
+    Jsd[c] = 0.5 * rJs * I2d[c];
   }
   const double w1 = 0.5 * gs;
-  const double w2 = -0.5 * gs / ((I2 + 1.0) * (I2 + 1.0));
+  const double w2 = -0.5 * gs * (rI * rI);
   double w2d[2];
 #pragma unroll
-  for (int c = 0; c < 2; ++c) w2d[c] = gs / ((I2 + 1.0) * (I2 + 1.0) * (I2 + 1.0)) * I2d[c];
+  for (int c = 0; c < 2; ++c) w2d[c] = gs * (rI * rI * rI) * I2d[c];
   double T[2][2], Td[2][2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      T[a][b] = 2.0 / Js * w1 * A[a][b] + 2.0 * Js * w2 * o.G[a][b];
+      T[a][b] = 2.0 * rJs * w1 * A[a][b] + 2.0 * Js * w2 * o.G[a][b];
 #pragma unroll
       for (int c = 0; c < 2; ++c)
-        Td[c][a][b] = -2.0 / (Js * Js) * Jsd[c] * w1 * A[a][b] + 2.0 / Js * w1 * Ad[c][a][b] +
+        Td[c][a][b] = -2.0 * (rJs * rJs) * Jsd[c] * w1 * A[a][b] + 2.0 * rJs * w1 * Ad[c][a][b] +
                       2.0 * Jsd[c] * w2 * o.G[a][b] + 2.0 * Js * w2d[c] * o.G[a][b] +
                       2.0 * Js * w2 * o.Gd[c][a][b];
     }

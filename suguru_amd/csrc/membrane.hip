@@ -77,6 +77,15 @@ __device__ inline void mul2(const double X[2][2], const double Y[2][2], double R
     for (int j = 0; j < 2; ++j) R[i][j] = X[i][0] * Y[0][j] + X[i][1] * Y[1][j];
 }
 
+// v of the adjacent lane (lanes 2k and 2k + 1 swap): DPP quad_perm [1, 0, 3, 2] on the two
+// halves, a VALU move instead of an LDS-crossbar permute; both lanes of a pair must be active
+__device__ __forceinline__ double swap_pair(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, 0xB1, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0xB1, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // X[c][q]: derivatives of coordinate c; returns the node geometry
 __device__ void geometry(const double (*X)[NQ], Geom& o) {
   double x1[2][3], x2[2][2][3], x3[2][2][2][3], x4[2][2][2][2][3];
@@ -550,7 +559,7 @@ __global__ void __launch_bounds__(THREADS, 2)
         for (int c = 0; c < 3; ++c)
 #pragma unroll
           for (int qq = 0; qq < NQ; ++qq) {
-            const double other = __shfl_xor(X[c][qq], 1);
+            const double other = swap_pair(X[c][qq]);
             X[c][qq] = par ? other - X[c][qq] : X[c][qq] + other;
           }
         Geom o;

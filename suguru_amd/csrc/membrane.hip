@@ -383,11 +383,10 @@ __device__ void law(const Geom& o, const double* __restrict__ R, double gs, doub
 // LDS: coarse coefficients [3][N][N] a and b, the longitude factors of both grids (2 (np + np2)
 // values), then a work region reused by the phases.
 // scratch: per workgroup, longitude DFT of the 2x-grid outputs [nt][6][N][2].
-// Registers: capped for two wavefronts per SIMD (256 VGPRs, about 40 of them spilled).  With
+// Registers: capped for two wavefronts per SIMD (256 VGPRs, about 42 of them spilled).  With
 // no cap the law takes 256 VGPRs + 40 AGPRs, one wavefront per SIMD, and runs 1.6x slower.
-// 128 threads per capsule and four capsules per CU beat 256 threads and two by 11 %: the
-// per-node phase takes RING x np nodes at a time (3 x 68 = 204 at L = 16), which one round of
-// 256 threads covers no better than two rounds of 128.
+// 128 threads per capsule and four capsules per CU (LDS: 39 KB each at L = 16); the per-node
+// phase takes chunks of 128 consecutive nodes, one per thread (step 2 below).
 // The transform loops carry `#pragma unroll 4`.  Their trip counts are run-time values, so
 // without the hint each iteration waits for its own table loads.  Each sum keeps its order,
 // so the output is bitwise unchanged; 1000 capsules at L = 16 take 1.32 instead of 1.47 ms
